@@ -1,0 +1,152 @@
+/*
+ * mhd_api.h - C ABI of the MI355X (gfx950) relaxing-TVD MHD timestep library
+ * (libmhdgpu.so).  Plain pointers and sizes only.
+ *
+ * Layout everywhere: the reference's Fortran-order AoS arrays
+ *   u[((k*ny+j)*nx+i)*5+c]  c = rho, rho vx, rho vy, rho vz, E   (cell centred)
+ *   b[((k*ny+j)*nx+i)*3+c]  c = bx, by, bz on the left faces of cell (i,j,k)
+ * periodic in all three directions (HIP/array_definition.h:5-7,
+ * Heterogeneous_MHD/C/array_definition.h:12-14, OpenMP/mhd.f90:35).
+ *
+ * Errors: every entry point returns MHD_OK (0) or a negative MHD_ERR_* code
+ * and never exits; mhd_last_error() gives the message of the calling thread's
+ * last failure.  The reference's wrappers return void and either exit(-1)
+ * (HIP/cuda_function.h:4-12) or ignore errors (HIP/hip_function.h:5-13).
+ */
+#ifndef MHD_API_H
+#define MHD_API_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MHD_OK 0
+#define MHD_ERR_ARG (-1)      /* bad pointer or argument */
+#define MHD_ERR_SHAPE (-2)    /* unsupported grid shape (pencil length n needs n%K==0, 64*K>=n, K<=8) */
+#define MHD_ERR_HIP (-3)      /* HIP runtime error (message in mhd_last_error) */
+#define MHD_ERR_NOMEM (-4)    /* device allocation failed or budget exceeded */
+#define MHD_ERR_STATE (-5)    /* call not valid in the object's current state */
+#define MHD_ERR_IO (-6)       /* file could not be read */
+#define MHD_ERR_FULL (-7)     /* request refused: queue full / over budget */
+#define MHD_ERR_NONFINITE (-8)/* non-finite value found where a finite one is required */
+
+const char *mhd_last_error(void);
+/* library version string, e.g. "mhdgpu 1.0 gfx950" */
+const char *mhd_version(void);
+
+/* ---- device memory plumbing for callers without a HIP toolchain --------- */
+int mhd_dev_alloc(void **ptr, size_t bytes);
+int mhd_dev_free(void *ptr);
+int mhd_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int mhd_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int mhd_memcpy_d2d(void *dst, const void *src, size_t bytes);
+int mhd_device_sync(void);
+/* free and total device memory in bytes */
+int mhd_mem_info(size_t *free_bytes, size_t *total_bytes);
+
+/* ---- step-level drop-ins (device pointers, FP32) ------------------------
+ * Each replaces one wrapper of HIP/hip_subroutine.h:4-12 (CUDA originals in
+ * HIP/cu/).  Arrays are device pointers in the frame the caller passes (the
+ * dims are the frame's, e.g. (ny,nx,nz) after transpose12, as in
+ * HIP/cu/cuda_main.cu:105); nx, ny, nz are HOST pointers (the reference also
+ * passes device copies d_nx.. for its kernels; here the library reads the
+ * host values).  dt is a DEVICE pointer, so a dt computed on the device is
+ * used without a round trip.  The calls are synchronous, like the reference's
+ * wrappers (cudaThreadSynchronize after each launch, e.g.
+ * HIP/cu/cuda_fluidx.cu:19). */
+
+/* replaces cuda_fluidx (HIP/cu/cuda_fluidx.cu:12, HIP/cuda_fluidx.cpp:13):
+ * one relaxing-TVD sweep along i of every (j,k) pencil, u updated in place */
+int mhd_fluidx(float *u, const float *b, const int *nx, const int *ny, const int *nz, const float *dt);
+
+/* replaces cuda_advectbyzx (HIP/cu/cuda_advectbyzx.cu:15, HIP/cuda_advectbyzx.cpp:16):
+ * constrained-transport advection of b2 then b3 along i, with the EMF
+ * correction of b1; the reference's N^3 scratch `temp` is not needed */
+int mhd_advectbyzx(const float *u, float *b, const int *nx, const int *ny, const int *nz, const float *dt);
+
+/* replaces cuda_transpose12 (HIP/cu/cuda_transpose12.cu:12, HIP/cuda_transpose12.cpp:13):
+ * (i,j,k)->(j,i,k) with u components (1,3,2,4,5) and b (2,1,3); the result
+ * is written to ut,bt and copied back into u,b as in the reference, so after
+ * the call u,b hold the (ny,nx,nz) frame */
+int mhd_transpose12(float *ut, float *bt, float *u, float *b, const int *nx, const int *ny, const int *nz);
+
+/* replaces cuda_transpose13 (HIP/cu/cuda_transpose13.cu:14, HIP/cuda_transpose13.cpp:15):
+ * (i,j,k)->(k,j,i) with u components (1,4,3,2,5) and b (3,2,1).  The
+ * reference supports only nx == nz (it prints "nz<>nx not supported" and
+ * leaves the state alone otherwise, cuda_transpose13.cu:27-42); this
+ * permutation is exact for any shape. */
+int mhd_transpose13(float *ut, float *bt, float *u, float *b, const int *nx, const int *ny, const int *nz);
+
+/* replaces cuda_cfl (HIP/cu/cuda_cfl.cu:14) and calcfl (Heterogeneous_MHD/C/calcfl.c:13):
+ * cfl = 1/max over cells of max|v| + sqrt(|2B^2 + gamma p|/rho), the C
+ * path's face-value B^2 (avg_b = 0) or the cell-averaged B^2 of
+ * OpenMP/mhd.f90:112-120 (avg_b = 1).  Written to *d_cfl (device) and, when
+ * h_cfl is not NULL, to *h_cfl (host).  The caller forms dt = 0.9*cfl as
+ * Heterogeneous_MHD/C/mhd.c:63 does (the GPU reference instead used 1/c as
+ * dt on the device, HIP/cu/cuda_cfl.cu:41-43). */
+int mhd_cfl(const float *u, const float *b, const int *nx, const int *ny, const int *nz, int avg_b, float *d_cfl,
+            float *h_cfl);
+
+/* ---- program-level drop-ins ---------------------------------------------- */
+
+/* replaces init (HIP/c_subroutine.h:4, Heterogeneous_MHD/C/init.c:11): read
+ * an initial-state file of 8 float32 per cell (u1..u5, b1..b3).  order 0 is
+ * the reference C reader (file's fastest axis lands on k, init.c:33-54);
+ * order 1 is plain Fortran order (x fastest, OpenMP_offloadMIC/work.f90:127-132).
+ * u (5 n^3) and b (3 n^3) are caller-owned host arrays; *nx,*ny,*nz give the
+ * grid (the reference fixes them at compile time). */
+int mhd_init(const char *path, int order, float *u, float *b, const int *nx, const int *ny, const int *nz);
+
+/* write the result files of Heterogeneous_MHD/C/mhd.c:126-163 (records in the
+ * order of the reader with order 0): 5 floats per cell to path_u, 3 to path_b */
+int mhd_write_result(const char *path_u, const char *path_b, const float *u, const float *b, const int *nx,
+                     const int *ny, const int *nz);
+
+/* replaces cuda_main (HIP/cu/cuda_main.cu:14, HIP/cuda_main.cpp:17): host
+ * arrays in and out, device-resident loop.  nsteps > 0 runs exactly nsteps
+ * loop iterations; nsteps <= 0 runs while t < tf like the reference
+ * (tf must then be finite).  dt = 0.9*cfl clamped to (tf-t)/2 as in
+ * Heterogeneous_MHD/C/mhd.c:63-65; pass tf = INFINITY for no clamp.  When
+ * h_dts is not NULL it receives the dt of each step (capacity max_dts). */
+int mhd_main(float *h_u, float *h_b, const int *nx, const int *ny, const int *nz, int nsteps, float tf,
+             float *h_dts, int max_dts, int *steps_done);
+
+/* ---- context API: one device-resident box ------------------------------- */
+typedef struct mhd_ctx mhd_ctx;
+
+/* precision: 4 (FP32, the reference's) or 8 (FP64).  avg_b as in mhd_cfl.
+ * Allocates 32 B (FP32) or 64 B (FP64) per cell on the current device. */
+int mhd_ctx_create(mhd_ctx **ctx, int nx, int ny, int nz, int precision, int avg_b);
+int mhd_ctx_destroy(mhd_ctx *ctx);
+/* host arrays in the ctx's precision (float* or double*) */
+int mhd_ctx_set_state(mhd_ctx *ctx, const void *h_u, const void *h_b);
+int mhd_ctx_get_state(mhd_ctx *ctx, void *h_u, void *h_b);
+/* tf for the dt clamp (default +inf); resets t, the dt history and the step count */
+int mhd_ctx_reset_clock(mhd_ctx *ctx, double tf);
+/* enqueue nsteps loop iterations on the ctx stream (a captured hipGraph
+ * replayed per step; dt stays on the device).  Asynchronous. */
+int mhd_ctx_step(mhd_ctx *ctx, int nsteps);
+/* same, bracketed by HIP events on the ctx stream; *ms = elapsed time */
+int mhd_ctx_step_timed(mhd_ctx *ctx, int nsteps, float *ms);
+int mhd_ctx_sync(mhd_ctx *ctx);
+/* dt history (in the ctx precision, as double) of the steps since the last
+ * reset, at most max; *count = steps taken */
+int mhd_ctx_dts(mhd_ctx *ctx, double *h_dts, int max, int *count);
+/* t as accumulated in the ctx precision and as an FP64 sum */
+int mhd_ctx_time(mhd_ctx *ctx, double *t, double *t64);
+/* FP64 monitors: out[6] = total mass, total energy, momentum x, y, z, max |div B| */
+int mhd_ctx_monitors(mhd_ctx *ctx, double *out6);
+/* device pointers of the state, for zero-copy callers */
+int mhd_ctx_device_state(mhd_ctx *ctx, void **d_u, void **d_b);
+/* time per kernel of one step, for the roofline: runs `reps` steps with
+ * events around every launch; ms_out[0..12] = cfl+dt, then the 12 sweeps in
+ * order (fluidx x, advect x, fluidx y, advect y, fluidx z, advect z,
+ * advect z, fluidx z, advect y, fluidx y, advect x, fluidx x), averaged */
+int mhd_ctx_profile_step(mhd_ctx *ctx, int reps, float *ms_out13);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

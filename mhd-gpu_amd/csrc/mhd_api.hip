@@ -1,0 +1,295 @@
+// C-ABI of libmhdgpu.so: step-level drop-ins, the program-level driver and
+// the device-resident context (include/mhd_api.h).
+#include "../../include/mhd_api.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "mhd_kernels.h"
+#include "mhd_internal.h"
+
+namespace mhd {
+
+static thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *where)
+{
+    return fail(MHD_ERR_HIP, "%s: %s", where, hipGetErrorString(e));
+}
+
+int check_shape(int nx, int ny, int nz)
+{
+    if (nx < 2 || ny < 1 || nz < 1) return fail(MHD_ERR_SHAPE, "grid %dx%dx%d: need nx>=2, ny>=1, nz>=1", nx, ny, nz);
+    if ((long)nx * ny * nz * 5 >= (1L << 31))
+        return fail(MHD_ERR_SHAPE, "grid %dx%dx%d: 5*cells must stay below 2^31", nx, ny, nz);
+    if (!pick_k(nx)) return fail(MHD_ERR_SHAPE, "pencil length %d unsupported (need n%%K==0, 64*K>=n, K<=8)", nx);
+    return MHD_OK;
+}
+
+// all three sweep directions of a box (the context path sweeps y and z too)
+int check_shape3(int nx, int ny, int nz)
+{
+    int rc = check_shape(nx, ny, nz);
+    if (rc) return rc;
+    if (ny < 2 || nz < 2) return fail(MHD_ERR_SHAPE, "grid %dx%dx%d: the full step needs ny, nz >= 2", nx, ny, nz);
+    if (!pick_k(ny) || !pick_k(nz))
+        return fail(MHD_ERR_SHAPE, "grid %dx%dx%d: pencil length unsupported (need n%%K==0, 64*K>=n, K<=8)", nx, ny,
+                    nz);
+    return MHD_OK;
+}
+
+// per-device scratch of the step-level entry points: the CFL max accumulator
+static std::mutex g_scr_mu;
+static unsigned long long *g_scr[64];
+
+static int step_scratch(unsigned long long **p)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= 64) return fail(MHD_ERR_ARG, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scr_mu);
+    if (!g_scr[dev]) {
+        e = hipMalloc((void **)&g_scr[dev], 64);
+        if (e != hipSuccess) return fail(MHD_ERR_NOMEM, "hipMalloc scratch: %s", hipGetErrorString(e));
+        launch_zero_u64(g_scr[dev], 8, 0);
+        e = hipDeviceSynchronize();
+        if (e != hipSuccess) return hip_fail(e, "scratch init");
+    }
+    *p = g_scr[dev];
+    return MHD_OK;
+}
+
+static int sync_check(hipError_t e, const char *where)
+{
+    if (e != hipSuccess) return hip_fail(e, where);
+    e = hipDeviceSynchronize();
+    if (e != hipSuccess) return hip_fail(e, where);
+    return MHD_OK;
+}
+
+}  // namespace mhd
+
+using namespace mhd;
+
+extern "C" {
+
+const char *mhd_last_error(void) { return g_err.c_str(); }
+const char *mhd_version(void) { return "mhdgpu 1.0 gfx950"; }
+
+int mhd_dev_alloc(void **ptr, size_t bytes)
+{
+    if (!ptr) return fail(MHD_ERR_ARG, "mhd_dev_alloc: null ptr");
+    hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+    if (e != hipSuccess) return fail(MHD_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    return MHD_OK;
+}
+
+int mhd_dev_free(void *ptr)
+{
+    hipError_t e = hipFree(ptr);
+    return e == hipSuccess ? MHD_OK : hip_fail(e, "hipFree");
+}
+
+int mhd_memcpy_h2d(void *dst, const void *src, size_t bytes)
+{
+    hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    return e == hipSuccess ? MHD_OK : hip_fail(e, "hipMemcpy H2D");
+}
+
+int mhd_memcpy_d2h(void *dst, const void *src, size_t bytes)
+{
+    hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+    return e == hipSuccess ? MHD_OK : hip_fail(e, "hipMemcpy D2H");
+}
+
+int mhd_memcpy_d2d(void *dst, const void *src, size_t bytes)
+{
+    hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice);
+    return e == hipSuccess ? MHD_OK : hip_fail(e, "hipMemcpy D2D");
+}
+
+int mhd_device_sync(void)
+{
+    hipError_t e = hipDeviceSynchronize();
+    return e == hipSuccess ? MHD_OK : hip_fail(e, "hipDeviceSynchronize");
+}
+
+int mhd_mem_info(size_t *free_bytes, size_t *total_bytes)
+{
+    size_t f = 0, t = 0;
+    hipError_t e = hipMemGetInfo(&f, &t);
+    if (e != hipSuccess) return hip_fail(e, "hipMemGetInfo");
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return MHD_OK;
+}
+
+int mhd_fluidx(float *u, const float *b, const int *nx, const int *ny, const int *nz, const float *dt)
+{
+    if (!u || !b || !nx || !ny || !nz || !dt) return fail(MHD_ERR_ARG, "mhd_fluidx: null argument");
+    int rc = check_shape(*nx, *ny, *nz);
+    if (rc) return rc;
+    return sync_check(launch_fluidx<float>(u, b, 0, *nx, *ny, *nz, dt, 0), "mhd_fluidx");
+}
+
+int mhd_advectbyzx(const float *u, float *b, const int *nx, const int *ny, const int *nz, const float *dt)
+{
+    if (!u || !b || !nx || !ny || !nz || !dt) return fail(MHD_ERR_ARG, "mhd_advectbyzx: null argument");
+    int rc = check_shape(*nx, *ny, *nz);
+    if (rc) return rc;
+    return sync_check(launch_advect<float>(u, b, 0, *nx, *ny, *nz, dt, 0), "mhd_advectbyzx");
+}
+
+static int transpose_common(int which, float *ut, float *bt, float *u, float *b, const int *nx, const int *ny,
+                            const int *nz)
+{
+    if (!ut || !bt || !u || !b || !nx || !ny || !nz) return fail(MHD_ERR_ARG, "mhd_transpose: null argument");
+    if (*nx < 1 || *ny < 1 || *nz < 1) return fail(MHD_ERR_SHAPE, "mhd_transpose: bad grid");
+    const size_t nc = (size_t)*nx * *ny * *nz;
+    hipError_t e = launch_transpose<float>(ut, bt, u, b, which, *nx, *ny, *nz, 0);
+    if (e != hipSuccess) return hip_fail(e, "mhd_transpose launch");
+    // copy back so that u,b hold the new frame (HIP/cu/cuda_transpose12.cu:33-34)
+    e = hipMemcpyAsync(u, ut, nc * 5 * sizeof(float), hipMemcpyDeviceToDevice, 0);
+    if (e == hipSuccess) e = hipMemcpyAsync(b, bt, nc * 3 * sizeof(float), hipMemcpyDeviceToDevice, 0);
+    return sync_check(e, "mhd_transpose");
+}
+
+int mhd_transpose12(float *ut, float *bt, float *u, float *b, const int *nx, const int *ny, const int *nz)
+{
+    return transpose_common(12, ut, bt, u, b, nx, ny, nz);
+}
+
+int mhd_transpose13(float *ut, float *bt, float *u, float *b, const int *nx, const int *ny, const int *nz)
+{
+    return transpose_common(13, ut, bt, u, b, nx, ny, nz);
+}
+
+int mhd_cfl(const float *u, const float *b, const int *nx, const int *ny, const int *nz, int avg_b, float *d_cfl,
+            float *h_cfl)
+{
+    if (!u || !b || !nx || !ny || !nz || !d_cfl) return fail(MHD_ERR_ARG, "mhd_cfl: null argument");
+    if (*nx < 1 || *ny < 1 || *nz < 1) return fail(MHD_ERR_SHAPE, "mhd_cfl: bad grid");
+    unsigned long long *scr = nullptr;
+    int rc = step_scratch(&scr);
+    if (rc) return rc;
+    hipError_t e = launch_cfl<float>(u, b, *nx, *ny, *nz, avg_b, scr, 0);
+    if (e == hipSuccess) e = launch_cfl_value<float>(scr, d_cfl, 0);
+    rc = sync_check(e, "mhd_cfl");
+    if (rc) return rc;
+    if (h_cfl) return mhd_memcpy_d2h(h_cfl, d_cfl, sizeof(float));
+    return MHD_OK;
+}
+
+int mhd_init(const char *path, int order, float *u, float *b, const int *nx, const int *ny, const int *nz)
+{
+    if (!path || !u || !b || !nx || !ny || !nz) return fail(MHD_ERR_ARG, "mhd_init: null argument");
+    const int X = *nx, Y = *ny, Z = *nz;
+    if (X < 1 || Y < 1 || Z < 1) return fail(MHD_ERR_SHAPE, "mhd_init: bad grid");
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return fail(MHD_ERR_IO, "mhd_init: cannot open %s", path);
+    const size_t nc = (size_t)X * Y * Z;
+    std::vector<float> rec(8 * nc);
+    const size_t got = fread(rec.data(), sizeof(float), rec.size(), fp);
+    fclose(fp);
+    if (got != rec.size()) return fail(MHD_ERR_IO, "mhd_init: %s holds %zu floats, need %zu", path, got, rec.size());
+    size_t r = 0;
+    if (order == 0) {  // i outer, k inner (Heterogeneous_MHD/C/init.c:33-54)
+        for (int i = 0; i < X; i++)
+            for (int j = 0; j < Y; j++)
+                for (int k = 0; k < Z; k++, r++) {
+                    const size_t s = ((size_t)k * Y + j) * X + i;
+                    for (int c = 0; c < 5; c++) u[5 * s + c] = rec[8 * r + c];
+                    for (int c = 0; c < 3; c++) b[3 * s + c] = rec[8 * r + 5 + c];
+                }
+    } else {  // x fastest
+        for (size_t s = 0; s < nc; s++) {
+            for (int c = 0; c < 5; c++) u[5 * s + c] = rec[8 * s + c];
+            for (int c = 0; c < 3; c++) b[3 * s + c] = rec[8 * s + 5 + c];
+        }
+    }
+    return MHD_OK;
+}
+
+int mhd_write_result(const char *path_u, const char *path_b, const float *u, const float *b, const int *nx,
+                     const int *ny, const int *nz)
+{
+    if (!path_u || !path_b || !u || !b || !nx || !ny || !nz) return fail(MHD_ERR_ARG, "mhd_write_result: null");
+    const int X = *nx, Y = *ny, Z = *nz;
+    FILE *fu = fopen(path_u, "wb");
+    FILE *fb = fopen(path_b, "wb");
+    if (!fu || !fb) {
+        if (fu) fclose(fu);
+        if (fb) fclose(fb);
+        return fail(MHD_ERR_IO, "mhd_write_result: cannot open output");
+    }
+    // Heterogeneous_MHD/C/mhd.c:130-161: first array index outermost
+    for (int i = 0; i < X; i++)
+        for (int j = 0; j < Y; j++)
+            for (int k = 0; k < Z; k++) {
+                const size_t s = ((size_t)k * Y + j) * X + i;
+                fwrite(u + 5 * s, sizeof(float), 5, fu);
+                fwrite(b + 3 * s, sizeof(float), 3, fb);
+            }
+    const int bad = ferror(fu) || ferror(fb);
+    fclose(fu);
+    fclose(fb);
+    return bad ? fail(MHD_ERR_IO, "mhd_write_result: write failed") : MHD_OK;
+}
+
+int mhd_main(float *h_u, float *h_b, const int *nx, const int *ny, const int *nz, int nsteps, float tf,
+             float *h_dts, int max_dts, int *steps_done)
+{
+    if (!h_u || !h_b || !nx || !ny || !nz) return fail(MHD_ERR_ARG, "mhd_main: null argument");
+    if (nsteps <= 0 && !isfinite(tf)) return fail(MHD_ERR_ARG, "mhd_main: run-to-tf needs a finite tf");
+    mhd_ctx *ctx = nullptr;
+    int rc = mhd_ctx_create(&ctx, *nx, *ny, *nz, 4, 0);
+    if (rc) return rc;
+    int done = 0;
+    rc = mhd_ctx_set_state(ctx, h_u, h_b);
+    if (!rc) rc = mhd_ctx_reset_clock(ctx, tf);
+    if (!rc) {
+        if (nsteps > 0) {
+            rc = mhd_ctx_step(ctx, nsteps);
+            done = nsteps;
+        } else {
+            // while (t < tf) (Heterogeneous_MHD/C/mhd.c:125, HIP/cu/cuda_main.cu:128)
+            double t = 0, t64 = 0;
+            while (!rc && t < (double)tf) {
+                rc = mhd_ctx_step(ctx, 1);
+                if (!rc) rc = mhd_ctx_time(ctx, &t, &t64);
+                done++;
+            }
+        }
+    }
+    if (!rc) rc = mhd_ctx_sync(ctx);
+    if (!rc) rc = mhd_ctx_get_state(ctx, h_u, h_b);
+    if (!rc && h_dts && max_dts > 0) {
+        std::vector<double> d(max_dts);
+        int cnt = 0;
+        rc = mhd_ctx_dts(ctx, d.data(), max_dts, &cnt);
+        for (int i = 0; i < cnt && i < max_dts; i++) h_dts[i] = (float)d[i];
+    }
+    if (steps_done) *steps_done = done;
+    const int rc2 = mhd_ctx_destroy(ctx);
+    return rc ? rc : rc2;
+}
+
+}  // extern "C"

@@ -1,0 +1,168 @@
+// Shared device code for the relaxing-TVD MHD kernels (gfx950).
+//
+// Numerics follow the reference C path (Heterogeneous_MHD/C) operation by
+// operation: FP32 arithmetic with the FP64 sub-expressions the C code gets
+// from double literals and fabs/sqrt, no FMA contraction (-ffp-contract=off),
+// IEEE FP32 division and square root, FP32 denormals kept.  With T = double
+// every operation is FP64 (the FP64 instantiation used as an error yardstick).
+//
+// The one liberty taken: the reference's helper a*X + b*Y + c
+// (Heterogeneous_MHD/C/includefunc_spu.h:73-94) is called with c = 0 and,
+// for the c*w products, b = 0; the "+ 0" and "0*w" terms only change the sign
+// of an exact zero result (and turn an infinity into NaN), so they are left
+// out.  Results therefore equal the reference's numerically (-0 == +0); every
+// non-zero value matches bit for bit.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mhd {
+
+// Pencil frame of a sweep over an x-major AoS box of nx*ny*nz cells.
+// The sweep runs along i; (j,k) number the pencils.  Frame 0 is the x sweep
+// (i,j,k) = (x,y,z); frame 1 the y sweep (i,j,k) = (y,x,z) with the reference's
+// transpose12 component relabelling (u: 1,3,2,4,5, b: 2,1,3); frame 2 the z
+// sweep (i,j,k) = (z,x,y), the state the reference reaches after transpose12
+// then transpose13 (u: 1,4,2,3,5, b: 3,1,2).
+struct Frame {
+    int ni, nj, nk;     // pencil length, pencils along j, pencils along k
+    long si, sj, sk;    // strides of i, j, k in cells
+};
+
+// Frame component maps: frame component c lives in physical component map[c].
+template <int FR> struct Comp;
+template <> struct Comp<0> {
+    static constexpr int u[5] = {0, 1, 2, 3, 4};
+    static constexpr int b[3] = {0, 1, 2};
+};
+template <> struct Comp<1> {
+    static constexpr int u[5] = {0, 2, 1, 3, 4};
+    static constexpr int b[3] = {1, 0, 2};
+};
+template <> struct Comp<2> {
+    static constexpr int u[5] = {0, 3, 1, 2, 4};
+    static constexpr int b[3] = {2, 0, 1};
+};
+// inverse maps: physical component p is frame component inv[p]
+template <int FR> struct CompInv;
+template <> struct CompInv<0> {
+    static constexpr int u[5] = {0, 1, 2, 3, 4};
+    static constexpr int b[3] = {0, 1, 2};
+};
+template <> struct CompInv<1> {
+    static constexpr int u[5] = {0, 2, 1, 3, 4};
+    static constexpr int b[3] = {1, 0, 2};
+};
+template <> struct CompInv<2> {
+    static constexpr int u[5] = {0, 2, 3, 1, 4};
+    static constexpr int b[3] = {1, 2, 0};
+};
+
+template <typename T> __device__ __forceinline__ T gamma_c() { return (T)(5.0 / 3.0); }
+
+// van Leer harmonic mean: the test is a product in T, the value
+// 2*p*m/(p+m) in double with p+m summed in T (tvd1.c:71-81, tvdb.c:72-76).
+template <typename T> __device__ __forceinline__ T vanleer(T p, T m)
+{
+    T d = (T)0;
+    if (p * m > (T)0) d = (T)(2.0 * (double)p * (double)m / (double)(p + m));
+    return d;
+}
+
+// ps of mhdflux.c:28 / calcfl.c:50, S = u2^2+u3^2+u4^2 and B = b1^2+b2^2+b3^2 in T
+template <typename T> __device__ __forceinline__ T pstar(T u1, T u5, T S, T B)
+{
+    const T g = gamma_c<T>();
+    return (T)(((double)u5 - (double)(S / u1) / 2.0) * ((double)g - 1.0) + (2.0 - (double)g) * (double)B / 2.0);
+}
+
+// One cell of mhdflux (mhdflux.c:21-46): unnormalised flux v[5] and the
+// freezing speed ci.
+template <typename T>
+__device__ __forceinline__ void mhdflux_cell(const T *u, const T *b, T *v, T &ci)
+{
+    const T u1 = u[0], u2 = u[1], u3 = u[2], u4 = u[3], u5 = u[4];
+    const T b1 = b[0], b2 = b[1], b3 = b[2];
+    const T vx = u2 / u1;
+    const T S = u2 * u2 + u3 * u3 + u4 * u4;
+    const T B = b1 * b1 + b2 * b2 + b3 * b3;
+    const T ps = pstar<T>(u1, u5, S, B);
+    v[0] = u2;
+    v[1] = u2 * vx + ps - b1 * b1;
+    v[2] = u3 * vx - b2 * b1;
+    v[3] = u4 * vx - b3 * b1;
+    v[4] = (u5 + ps) * vx - b1 * (b1 * u2 + b2 * u3 + b3 * u4) / u1;
+    const T p = (T)((double)ps - (double)B / 2.0);
+    const T g = gamma_c<T>();
+    ci = (T)(fabs((double)vx) + sqrt(fabs((double)((B + g * p) / u1))));
+}
+
+// Per-cell fast speed of calcfl.c:45-54 (face B at the cell, or the cell
+// average of OpenMP/mhd.f90:112-120 when the caller passes averaged b).
+template <typename T>
+__device__ __forceinline__ T cfl_speed(T u1, T u2, T u3, T u4, T u5, T B)
+{
+    const T g = gamma_c<T>();
+    const T t1 = (T)fabs((double)(u2 / u1)), t2 = (T)fabs((double)(u3 / u1)), t3 = (T)fabs((double)(u4 / u1));
+    const T v = t1 > t2 ? (t1 > t3 ? t1 : t3) : (t2 > t3 ? t2 : t3);
+    const T S = u2 * u2 + u3 * u3 + u4 * u4;
+    const T ps = pstar<T>(u1, u5, S, B);
+    const T p = (T)((double)ps - (double)B / 2.0);
+    return (T)((double)v + sqrt(fabs((double)B * 2.0 + (double)(g * p)) / (double)u1));
+}
+
+// ---- wave-level helpers: a pencil of n = L*K cells lives in one wave, lane
+// l < L owning cells l*K .. l*K+K-1 in registers. ----
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+template <typename T> __device__ __forceinline__ T shfl_from(T x, int src)
+{
+    if constexpr (sizeof(T) == 4) {
+        return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(x)));
+    } else {
+        const long long v = __double_as_longlong(x);
+        const int lo = __builtin_amdgcn_ds_bpermute(src << 2, (int)(v & 0xffffffff));
+        const int hi = __builtin_amdgcn_ds_bpermute(src << 2, (int)(v >> 32));
+        return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    }
+}
+
+// a[q] <- a[q+1] along the periodic pencil (cshift by +1)
+template <typename T, int K>
+__device__ __forceinline__ void shift_next(const T (&a)[K], T (&o)[K], int src_next)
+{
+#pragma unroll
+    for (int q = 0; q < K - 1; q++) o[q] = a[q + 1];
+    o[K - 1] = shfl_from(a[0], src_next);
+}
+
+// a[q] <- a[q-1] along the periodic pencil (cshift by -1)
+template <typename T, int K>
+__device__ __forceinline__ void shift_prev(const T (&a)[K], T (&o)[K], int src_prev)
+{
+    o[0] = shfl_from(a[K - 1], src_prev);
+#pragma unroll
+    for (int q = 1; q < K; q++) o[q] = a[q - 1];
+}
+
+// Max over the 64 lanes (callers pass 0 from inactive lanes; all values >= 0).
+// fmax skips NaN, like find_max's strict > (mhdflux.c:66-69).
+template <typename T> __device__ __forceinline__ T wave_max(T x)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) x = fmax(x, shfl_from(x, lane_id() ^ off));
+    return x;
+}
+
+// Block index -> tile index so that each XCD (blocks are dealt round-robin
+// over the 8 XCDs) receives a contiguous range of tiles: neighbouring tiles
+// share cache lines and neighbour pencils in the same L2.
+__device__ __forceinline__ int xcd_tile(int bid, int nb)
+{
+    const int q = nb >> 3, r = nb & 7;
+    const int x = bid & 7, idx = bid >> 3;
+    return x < r ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
+}
+
+}  // namespace mhd

@@ -1,0 +1,343 @@
+// Device-resident timestep driver (the mhd_ctx_* entry points).
+//
+// One loop iteration of the reference (Heterogeneous_MHD/C/mhd.c:61-119,
+// HIP/cu/cuda_main.cu:89-121) is 1 cfl reduction + 1 dt kernel + 12 sweeps.
+// The reference transposes the whole state four times per step so that
+// every sweep runs along the contiguous axis; here each sweep reads its
+// pencils in place through a frame (strides plus the transposes' component
+// relabelling), so the transposes and their 8 copies disappear.  dt is
+// computed and consumed on the device; the step is captured once into a
+// hipGraph and replayed.
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/mhd_api.h"
+#include "mhd_internal.h"
+
+using namespace mhd;
+
+struct mhd_ctx {
+    int nx, ny, nz, prec, avg;
+    size_t ncell;
+    hipStream_t st;
+    void *u, *b;               // state (AoS, ctx precision)
+    unsigned long long *cmax;  // CFL max accumulator
+    void *dt, *t;              // device scalars in ctx precision
+    double *t64;
+    void *dts;                 // dt history
+    int cap;
+    int *iter;
+    double *mon_part, *mon_out;
+    int mon_nblk;
+    double tf;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    bool captured;
+    hipEvent_t ev0, ev1;
+};
+
+static const int kDtCap = 1 << 20;
+
+template <typename T> static hipError_t enqueue_cfl_dt(mhd_ctx *c)
+{
+    hipError_t e = launch_cfl<T>((const T *)c->u, (const T *)c->b, c->nx, c->ny, c->nz, c->avg, c->cmax, c->st);
+    if (e == hipSuccess)
+        e = launch_dt<T>(c->cmax, (T *)c->dt, (T *)c->t, c->t64, (T)c->tf, (T *)c->dts, c->iter, c->cap, c->st);
+    return e;
+}
+
+// sweep s of the Strang sequence (HIP/cu/cuda_main.cu:99-121): frame and op
+static const int kSweepFrame[12] = {0, 0, 1, 1, 2, 2, 2, 2, 1, 1, 0, 0};
+static const int kSweepOp[12] = {0, 1, 0, 1, 0, 1, 1, 0, 1, 0, 1, 0};  // 0 fluidx, 1 advectbyzx
+
+template <typename T> static hipError_t enqueue_sweep(mhd_ctx *c, int s)
+{
+    const int fr = kSweepFrame[s];
+    if (kSweepOp[s] == 0)
+        return launch_fluidx<T>((T *)c->u, (const T *)c->b, fr, c->nx, c->ny, c->nz, (const T *)c->dt, c->st);
+    return launch_advect<T>((const T *)c->u, (T *)c->b, fr, c->nx, c->ny, c->nz, (const T *)c->dt, c->st);
+}
+
+template <typename T> static hipError_t enqueue_step(mhd_ctx *c)
+{
+    hipError_t e = enqueue_cfl_dt<T>(c);
+    for (int s = 0; s < 12 && e == hipSuccess; s++) e = enqueue_sweep<T>(c, s);
+    return e;
+}
+
+static hipError_t enqueue_step_any(mhd_ctx *c)
+{
+    return c->prec == 4 ? enqueue_step<float>(c) : enqueue_step<double>(c);
+}
+
+static int ensure_graph(mhd_ctx *c)
+{
+    if (c->captured) return MHD_OK;
+    hipError_t e = hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+    hipError_t el = enqueue_step_any(c);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(c->st, &g);
+    if (el != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return hip_fail(el, "capture of the step");
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    c->graph = g;
+    c->exec = x;
+    c->captured = true;
+    return MHD_OK;
+}
+
+static void drop_graph(mhd_ctx *c)
+{
+    if (c->exec) (void)hipGraphExecDestroy(c->exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->exec = nullptr;
+    c->graph = nullptr;
+    c->captured = false;
+}
+
+extern "C" {
+
+int mhd_ctx_create(mhd_ctx **out, int nx, int ny, int nz, int precision, int avg_b)
+{
+    if (!out) return fail(MHD_ERR_ARG, "mhd_ctx_create: null out");
+    *out = nullptr;
+    if (precision != 4 && precision != 8) return fail(MHD_ERR_ARG, "precision must be 4 or 8");
+    int rc = check_shape3(nx, ny, nz);
+    if (rc) return rc;
+    mhd_ctx *c = new mhd_ctx();
+    memset(c, 0, sizeof *c);
+    c->nx = nx;
+    c->ny = ny;
+    c->nz = nz;
+    c->prec = precision;
+    c->avg = avg_b ? 1 : 0;
+    c->ncell = (size_t)nx * ny * nz;
+    c->tf = INFINITY;
+    c->cap = kDtCap;
+    c->mon_nblk = 1024;
+    const size_t P = precision;
+    hipError_t e = hipStreamCreate(&c->st);
+    if (e != hipSuccess) {
+        delete c;
+        return hip_fail(e, "hipStreamCreate");
+    }
+    struct A {
+        void **p;
+        size_t n;
+    } al[] = {{&c->u, 5 * c->ncell * P},   {&c->b, 3 * c->ncell * P}, {(void **)&c->cmax, 64},
+              {&c->dt, 64},                 {&c->t, 64},               {(void **)&c->t64, 64},
+              {&c->dts, (size_t)kDtCap * P}, {(void **)&c->iter, 64},  {(void **)&c->mon_part, 6 * 8 * 1024},
+              {(void **)&c->mon_out, 64}};
+    for (auto &a : al) {
+        e = hipMalloc(a.p, a.n);
+        if (e != hipSuccess) {
+            mhd_ctx_destroy(c);
+            return fail(MHD_ERR_NOMEM, "mhd_ctx_create: hipMalloc(%zu): %s", a.n, hipGetErrorString(e));
+        }
+    }
+    e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e == hipSuccess) e = c->prec == 4 ? prepare_sweeps<float>(nx, ny, nz) : prepare_sweeps<double>(nx, ny, nz);
+    if (e != hipSuccess) {
+        mhd_ctx_destroy(c);
+        return hip_fail(e, "mhd_ctx_create");
+    }
+    rc = mhd_ctx_reset_clock(c, INFINITY);
+    if (rc) {
+        mhd_ctx_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return MHD_OK;
+}
+
+int mhd_ctx_destroy(mhd_ctx *c)
+{
+    if (!c) return MHD_OK;
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    drop_graph(c);
+    void *ps[] = {c->u, c->b, c->cmax, c->dt, c->t, c->t64, c->dts, c->iter, c->mon_part, c->mon_out};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+    return MHD_OK;
+}
+
+int mhd_ctx_set_state(mhd_ctx *c, const void *h_u, const void *h_b)
+{
+    if (!c || !h_u || !h_b) return fail(MHD_ERR_ARG, "mhd_ctx_set_state: null argument");
+    hipError_t e = hipMemcpyAsync(c->u, h_u, 5 * c->ncell * c->prec, hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->b, h_b, 3 * c->ncell * c->prec, hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    return e == hipSuccess ? MHD_OK : hip_fail(e, "mhd_ctx_set_state");
+}
+
+int mhd_ctx_get_state(mhd_ctx *c, void *h_u, void *h_b)
+{
+    if (!c || !h_u || !h_b) return fail(MHD_ERR_ARG, "mhd_ctx_get_state: null argument");
+    hipError_t e = hipMemcpyAsync(h_u, c->u, 5 * c->ncell * c->prec, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(h_b, c->b, 3 * c->ncell * c->prec, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    return e == hipSuccess ? MHD_OK : hip_fail(e, "mhd_ctx_get_state");
+}
+
+int mhd_ctx_reset_clock(mhd_ctx *c, double tf)
+{
+    if (!c) return fail(MHD_ERR_ARG, "mhd_ctx_reset_clock: null ctx");
+    hipError_t e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) return hip_fail(e, "mhd_ctx_reset_clock");
+    // tf is baked into the captured dt kernel: capture again on the next step
+    drop_graph(c);
+    c->tf = tf;
+    e = hipMemsetAsync(c->cmax, 0, 64, c->st);
+    if (e == hipSuccess) e = hipMemsetAsync(c->t, 0, 64, c->st);
+    if (e == hipSuccess) e = hipMemsetAsync(c->dt, 0, 64, c->st);
+    if (e == hipSuccess) e = hipMemsetAsync(c->t64, 0, 64, c->st);
+    if (e == hipSuccess) e = hipMemsetAsync(c->iter, 0, 64, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    return e == hipSuccess ? MHD_OK : hip_fail(e, "mhd_ctx_reset_clock");
+}
+
+int mhd_ctx_step(mhd_ctx *c, int nsteps)
+{
+    if (!c || nsteps < 0) return fail(MHD_ERR_ARG, "mhd_ctx_step: bad argument");
+    int rc = ensure_graph(c);
+    if (rc) return rc;
+    for (int s = 0; s < nsteps; s++) {
+        hipError_t e = hipGraphLaunch(c->exec, c->st);
+        if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    }
+    return MHD_OK;
+}
+
+int mhd_ctx_step_timed(mhd_ctx *c, int nsteps, float *ms)
+{
+    if (!c || !ms) return fail(MHD_ERR_ARG, "mhd_ctx_step_timed: null argument");
+    int rc = ensure_graph(c);
+    if (rc) return rc;
+    hipError_t e = hipEventRecord(c->ev0, c->st);
+    for (int s = 0; s < nsteps && e == hipSuccess; s++) e = hipGraphLaunch(c->exec, c->st);
+    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->st);
+    if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, c->ev0, c->ev1);
+    return e == hipSuccess ? MHD_OK : hip_fail(e, "mhd_ctx_step_timed");
+}
+
+int mhd_ctx_sync(mhd_ctx *c)
+{
+    if (!c) return fail(MHD_ERR_ARG, "mhd_ctx_sync: null ctx");
+    hipError_t e = hipStreamSynchronize(c->st);
+    return e == hipSuccess ? MHD_OK : hip_fail(e, "mhd_ctx_sync");
+}
+
+int mhd_ctx_dts(mhd_ctx *c, double *h_dts, int max, int *count)
+{
+    if (!c || !count) return fail(MHD_ERR_ARG, "mhd_ctx_dts: null argument");
+    int it = 0;
+    hipError_t e = hipStreamSynchronize(c->st);
+    if (e == hipSuccess) e = hipMemcpy(&it, c->iter, sizeof(int), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "mhd_ctx_dts");
+    *count = it;
+    int n = it < max ? it : max;
+    if (n > c->cap) n = c->cap;
+    if (n > 0 && h_dts) {
+        std::vector<unsigned char> raw((size_t)n * c->prec);
+        e = hipMemcpy(raw.data(), c->dts, raw.size(), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(e, "mhd_ctx_dts copy");
+        for (int i = 0; i < n; i++)
+            h_dts[i] = c->prec == 4 ? (double)((const float *)raw.data())[i] : ((const double *)raw.data())[i];
+    }
+    return MHD_OK;
+}
+
+int mhd_ctx_time(mhd_ctx *c, double *t, double *t64)
+{
+    if (!c) return fail(MHD_ERR_ARG, "mhd_ctx_time: null ctx");
+    unsigned char raw[8] = {0};
+    double d64 = 0;
+    hipError_t e = hipStreamSynchronize(c->st);
+    if (e == hipSuccess) e = hipMemcpy(raw, c->t, c->prec, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(&d64, c->t64, 8, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "mhd_ctx_time");
+    if (t) {
+        if (c->prec == 4) {
+            float f;
+            memcpy(&f, raw, 4);
+            *t = f;
+        } else {
+            memcpy(t, raw, 8);
+        }
+    }
+    if (t64) *t64 = d64;
+    return MHD_OK;
+}
+
+int mhd_ctx_monitors(mhd_ctx *c, double *out6)
+{
+    if (!c || !out6) return fail(MHD_ERR_ARG, "mhd_ctx_monitors: null argument");
+    hipError_t e = c->prec == 4 ? launch_monitors<float>((const float *)c->u, (const float *)c->b, c->nx, c->ny, c->nz,
+                                                          c->mon_part, c->mon_nblk, c->mon_out, c->st)
+                                : launch_monitors<double>((const double *)c->u, (const double *)c->b, c->nx, c->ny,
+                                                          c->nz, c->mon_part, c->mon_nblk, c->mon_out, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(out6, c->mon_out, 6 * sizeof(double), hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    return e == hipSuccess ? MHD_OK : hip_fail(e, "mhd_ctx_monitors");
+}
+
+int mhd_ctx_device_state(mhd_ctx *c, void **d_u, void **d_b)
+{
+    if (!c) return fail(MHD_ERR_ARG, "mhd_ctx_device_state: null ctx");
+    if (d_u) *d_u = c->u;
+    if (d_b) *d_b = c->b;
+    return MHD_OK;
+}
+
+int mhd_ctx_profile_step(mhd_ctx *c, int reps, float *ms13)
+{
+    if (!c || !ms13 || reps < 1) return fail(MHD_ERR_ARG, "mhd_ctx_profile_step: bad argument");
+    hipEvent_t ev[14];
+    for (int i = 0; i < 14; i++) {
+        hipError_t e = hipEventCreate(&ev[i]);
+        if (e != hipSuccess) {
+            for (int j = 0; j < i; j++) (void)hipEventDestroy(ev[j]);
+            return hip_fail(e, "hipEventCreate");
+        }
+    }
+    double acc[13] = {0};
+    hipError_t e = hipSuccess;
+    for (int r = 0; r < reps && e == hipSuccess; r++) {
+        e = hipEventRecord(ev[0], c->st);
+        if (e == hipSuccess) e = c->prec == 4 ? enqueue_cfl_dt<float>(c) : enqueue_cfl_dt<double>(c);
+        if (e == hipSuccess) e = hipEventRecord(ev[1], c->st);
+        for (int s = 0; s < 12 && e == hipSuccess; s++) {
+            e = c->prec == 4 ? enqueue_sweep<float>(c, s) : enqueue_sweep<double>(c, s);
+            if (e == hipSuccess) e = hipEventRecord(ev[s + 2], c->st);
+        }
+        if (e == hipSuccess) e = hipEventSynchronize(ev[13]);
+        for (int s = 0; s < 13 && e == hipSuccess; s++) {
+            float ms = 0;
+            e = hipEventElapsedTime(&ms, ev[s], ev[s + 1]);
+            acc[s] += ms;
+        }
+    }
+    for (int i = 0; i < 14; i++) (void)hipEventDestroy(ev[i]);
+    if (e != hipSuccess) return hip_fail(e, "mhd_ctx_profile_step");
+    for (int s = 0; s < 13; s++) ms13[s] = (float)(acc[s] / reps);
+    return MHD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,18 @@
+// Internal helpers shared by the C-ABI translation units.
+#pragma once
+#include <stdarg.h>
+
+#include "mhd_kernels.h"
+
+namespace mhd {
+// record the calling thread's error message and return code
+int fail(int code, const char *fmt, ...);
+int hip_fail(hipError_t e, const char *where);
+// MHD_OK when every sweep of an nx*ny*nz box has a supported pencil length
+int check_shape(int nx, int ny, int nz);
+// the same for all three sweep directions of a box
+int check_shape3(int nx, int ny, int nz);
+// set the dynamic-LDS limit of the sweep kernels a box will use, so that no
+// attribute call happens inside a stream capture
+template <typename T> hipError_t prepare_sweeps(int nx, int ny, int nz);
+}  // namespace mhd

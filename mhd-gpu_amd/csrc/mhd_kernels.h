@@ -1,0 +1,40 @@
+// Host-side launchers of the gfx950 timestep kernels (mhd_kernels.hip).
+// All launches are stream-ordered; T is float (the reference's precision and
+// evaluation order) or double (the FP64 instantiation).
+#pragma once
+#include "mhd_common.h"
+
+namespace mhd {
+
+// smallest supported cells-per-lane K with 64*K >= n and n % K == 0; 0 if none
+int pick_k(int n);
+Frame make_frame(int fr, int nx, int ny, int nz);
+
+// sweeps over an x-major AoS box nx*ny*nz in frame fr (0: x, 1: y, 2: z)
+template <typename T>
+hipError_t launch_fluidx(T *u, const T *b, int fr, int nx, int ny, int nz, const T *dt, hipStream_t st);
+template <typename T>
+hipError_t launch_advect(const T *u, T *b, int fr, int nx, int ny, int nz, const T *dt, hipStream_t st);
+
+// per-cell CFL speed max into *cmax (IEEE bits, atomically max-ed)
+template <typename T>
+hipError_t launch_cfl(const T *u, const T *b, int nx, int ny, int nz, int avg, unsigned long long *cmax,
+                      hipStream_t st);
+// dt, t, dt history from *cmax; clears *cmax
+template <typename T>
+hipError_t launch_dt(unsigned long long *cmax, T *dt, T *t, double *t64, T tf, T *dts, int *iter, int cap,
+                     hipStream_t st);
+template <typename T> hipError_t launch_cfl_value(unsigned long long *cmax, T *out, hipStream_t st);
+template <typename T> hipError_t prepare_sweeps(int nx, int ny, int nz);
+hipError_t launch_zero_u64(unsigned long long *p, int n, hipStream_t st);
+
+// which = 12 or 13: out-of-place transposes with the reference's component maps
+template <typename T>
+hipError_t launch_transpose(T *uo, T *bo, const T *u, const T *b, int which, int nx, int ny, int nz, hipStream_t st);
+
+// FP64 monitors: out[6] = mass, energy, momentum x, y, z, max |div B|
+template <typename T>
+hipError_t launch_monitors(const T *u, const T *b, int nx, int ny, int nz, double *part, int nblk, double *out,
+                           hipStream_t st);
+
+}  // namespace mhd

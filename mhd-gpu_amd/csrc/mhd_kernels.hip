@@ -1,0 +1,851 @@
+// gfx950 kernels of the relaxing-TVD MHD timestep.
+//
+// Work decomposition (all sweeps): one workgroup owns a tile of W pencils
+// that are neighbours along j (the next pencil along j is the fluid sweep's
+// "jp" and the CT sweep's "j+1"), one wavefront per pencil, each lane holding
+// K consecutive cells of the pencil in registers (n = L*K, L <= 64 lanes).
+// The tile is staged through LDS with coalesced loads: for the x frame a
+// pencil is a contiguous run of n cells, for the y and z frames (sj == 1) each
+// row i of the tile is a contiguous run of W cells.  LDS groups of K cells are
+// padded to an odd word count so that lane-strided reads are conflict free.
+// Neighbours along the pencil come from lane shuffles; the pencil-global max
+// of the freezing speed is a wave reduction.
+#include "mhd_kernels.h"
+
+namespace mhd {
+
+// padded LDS offset of cell i, component c, in a pencil of C-component cells
+template <int K, int C> __device__ __forceinline__ int pidx(int i, int c)
+{
+    return (i / K) * (C * K + 1) + (i % K) * C + c;
+}
+template <int K, int C> __host__ __device__ constexpr int plen(int n) { return (n / K) * (C * K + 1); }
+
+// j >= 0 reduced mod nj (j < nj + W + 2 in every caller, so the loop runs at
+// most a few times even for boxes narrower than a tile)
+__device__ __forceinline__ int wrapj(int j, int nj)
+{
+    while (j >= nj) j -= nj;
+    return j;
+}
+
+// Load all C components of nslot pencils (j = j0+s mod nj, plane k) into LDS
+// slot s (pitch P), physical component p stored as frame component inv[p].
+template <typename T, int K, int C, int S, bool XMAJ, class Inv>
+__device__ __forceinline__ void load_full(T *lds, int P, const T *__restrict__ g, const Frame &f, int j0,
+                                          int nslot, int k)
+{
+    const int n = f.ni;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    if constexpr (XMAJ) {
+        for (int s = 0; s < nslot; s++) {
+            const long base = ((long)wrapj(j0 + s, f.nj) * f.sj + (long)k * f.sk) * C;
+            for (int e = tid; e < n * C; e += nt) {
+                const int i = e / C, p = e - i * C;
+                lds[s * P + pidx<K, C>(i, Inv::at(p))] = g[base + e];
+            }
+        }
+    } else {
+        const long kb = (long)k * f.sk;
+        for (int e = tid; e < n * S * C; e += nt) {
+            const int i = e / (S * C), r = e - i * (S * C);
+            const int s = r / C, p = r - s * C;
+            if (s < nslot) {
+                const long cell = kb + (long)i * f.si + wrapj(j0 + s, f.nj);
+                lds[s * P + pidx<K, C>(i, Inv::at(p))] = g[cell * C + p];
+            }
+        }
+    }
+}
+
+// Store frame components back (inverse of load_full), nslot pencils.
+template <typename T, int K, int C, int S, bool XMAJ, class Inv>
+__device__ __forceinline__ void store_full(const T *lds, int P, T *__restrict__ g, const Frame &f, int j0,
+                                           int nslot, int k)
+{
+    const int n = f.ni;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    if constexpr (XMAJ) {
+        for (int s = 0; s < nslot; s++) {
+            const long base = ((long)(j0 + s) * f.sj + (long)k * f.sk) * C;
+            for (int e = tid; e < n * C; e += nt) {
+                const int i = e / C, p = e - i * C;
+                g[base + e] = lds[s * P + pidx<K, C>(i, Inv::at(p))];
+            }
+        }
+    } else {
+        const long kb = (long)k * f.sk;
+        for (int e = tid; e < n * S * C; e += nt) {
+            const int i = e / (S * C), r = e - i * (S * C);
+            const int s = r / C, p = r - s * C;
+            if (s < nslot) {
+                const long cell = kb + (long)i * f.si + (j0 + s);
+                g[cell * C + p] = lds[s * P + pidx<K, C>(i, Inv::at(p))];
+            }
+        }
+    }
+}
+
+// Load one physical component p (of C) of nslot pencils starting at pencil
+// j0 (mod nj) in plane k, single-word cells, pitch P.
+template <typename T, int K, int C, int S, bool XMAJ>
+__device__ __forceinline__ void load_comp(T *lds, int P, const T *__restrict__ g, int p, const Frame &f, int j0,
+                                          int nslot, int k)
+{
+    const int n = f.ni;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const long kb = (long)k * f.sk;
+    if constexpr (XMAJ) {
+        for (int s = 0; s < nslot; s++) {
+            const long base = (long)wrapj(j0 + s, f.nj) * f.sj + kb;
+            for (int i = tid; i < n; i += nt) lds[s * P + pidx<K, 1>(i, 0)] = g[(base + i) * C + p];
+        }
+    } else {
+        for (int e = tid; e < n * S; e += nt) {
+            const int i = e / S, s = e - i * S;
+            if (s < nslot) {
+                const long cell = kb + (long)i * f.si + wrapj(j0 + s, f.nj);
+                lds[s * P + pidx<K, 1>(i, 0)] = g[cell * C + p];
+            }
+        }
+    }
+}
+
+// physical component p -> frame component, as selects (p varies per lane)
+template <int FR> struct InvU {
+    static __device__ __forceinline__ int at(int p)
+    {
+        return p == 0 ? CompInv<FR>::u[0]
+                      : p == 1 ? CompInv<FR>::u[1]
+                               : p == 2 ? CompInv<FR>::u[2] : p == 3 ? CompInv<FR>::u[3] : CompInv<FR>::u[4];
+    }
+};
+template <int FR> struct InvB {
+    static __device__ __forceinline__ int at(int p)
+    {
+        return p == 0 ? CompInv<FR>::b[0] : p == 1 ? CompInv<FR>::b[1] : CompInv<FR>::b[2];
+    }
+};
+
+// ---------------------------------------------------------------------------
+// fluidx: Heterogeneous_MHD/C/fluidx.c:19-102 + tvd1.c:8-118 + mhdflux.c.
+// ---------------------------------------------------------------------------
+template <typename T, int K, int W> __host__ __device__ constexpr int fluidx_lds_words(int n)
+{
+    return W * plen<K, 5>(n) + (W + 1) * plen<K, 3>(n) + W * plen<K, 1>(n);
+}
+
+template <typename T, int K>
+__device__ __forceinline__ void mhdflux_pencil(const T (&uu)[K][5], const T (&bx)[K][3], T (&v)[K][5], T &c)
+{
+    T cl = (T)0;
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+        T ci;
+        mhdflux_cell<T>(uu[q], bx[q], v[q], ci);
+        cl = fmax(cl, ci);
+    }
+    c = wave_max(cl);
+    if (c > (T)0) {
+#pragma unroll
+        for (int q = 0; q < K; q++)
+#pragma unroll
+            for (int m = 0; m < 5; m++) v[q][m] = v[q][m] / c;
+    }
+}
+
+// tvd1 on a pencil held K cells per lane; uu is updated in place.
+template <typename T, int K>
+__device__ __forceinline__ void tvd1_pencil(T (&uu)[K][5], const T (&bx)[K][3], T dt, int src_next, int src_prev)
+{
+    const T h = (T)(-1.0 * (double)dt / 2.0);
+    const T mdt = (T)(-1.0 * (double)dt);
+    T v[K][5], c;
+    T u1[K][5];
+    // first half step (tvd1.c:34-49)
+    mhdflux_pencil<T, K>(uu, bx, v, c);
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        T wl[K], fr[K], wln[K], fl[K], fx[K], fxp[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            const T wr = uu[q][m] + v[q][m];
+            wl[q] = uu[q][m] - v[q][m];
+            fr[q] = c * wr;
+        }
+        shift_next<T, K>(wl, wln, src_next);
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            fl[q] = c * wln[q];
+            fx[q] = (T)0.5 * fr[q] + (T)(-0.5) * fl[q];
+        }
+        shift_prev<T, K>(fx, fxp, src_prev);
+#pragma unroll
+        for (int q = 0; q < K; q++) u1[q][m] = uu[q][m] + h * (fx[q] - fxp[q]);
+    }
+    // full step with limited fluxes (tvd1.c:52-116)
+    mhdflux_pencil<T, K>(u1, bx, v, c);
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        T wl[K], fr[K], frn[K], frp[K], wln[K], fl[K], fln[K], flp[K], fx[K], fxp[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            const T wr = u1[q][m] + v[q][m];
+            wl[q] = u1[q][m] - v[q][m];
+            fr[q] = c * wr;
+        }
+        shift_next<T, K>(fr, frn, src_next);
+        shift_prev<T, K>(fr, frp, src_prev);
+        shift_next<T, K>(wl, wln, src_next);
+#pragma unroll
+        for (int q = 0; q < K; q++) fl[q] = c * wln[q];
+        shift_next<T, K>(fl, fln, src_next);
+        shift_prev<T, K>(fl, flp, src_prev);
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            const T dfrp = (T)0.5 * frn[q] + (T)(-0.5) * fr[q];
+            const T dfrm = (T)0.5 * fr[q] + (T)(-0.5) * frp[q];
+            const T dflp = (T)0.5 * fl[q] + (T)(-0.5) * fln[q];
+            const T dflm = (T)0.5 * flp[q] + (T)(-0.5) * fl[q];
+            const T dfr = vanleer<T>(dfrp, dfrm);
+            const T dfl = vanleer<T>(dflp, dflm);
+            fx[q] = (T)0.5 * (fr[q] - fl[q]) + (T)0.5 * (dfr - dfl);
+        }
+        shift_prev<T, K>(fx, fxp, src_prev);
+#pragma unroll
+        for (int q = 0; q < K; q++) uu[q][m] = uu[q][m] + mdt * (fx[q] - fxp[q]);
+    }
+}
+
+template <typename T, int K, int W, int FR>
+__global__ void __launch_bounds__(64 * W) fluidx_kernel(T *__restrict__ u, const T *__restrict__ b, Frame f,
+                                                        const T *__restrict__ dtp, int ntj)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *smem = reinterpret_cast<T *>(smem_raw);
+    constexpr bool XM = (FR == 0);
+    const int n = f.ni, L = n / K;
+    const int PU = plen<K, 5>(n), PB = plen<K, 3>(n), PK = plen<K, 1>(n);
+    T *su = smem, *sb = su + W * PU, *sk = sb + (W + 1) * PB;
+
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int k = tile / ntj, j0 = (tile - k * ntj) * W;
+    const int nvalid = min(W, f.nj - j0);
+    const int kp = (k + 1 == f.nk) ? 0 : k + 1;
+
+    load_full<T, K, 5, W, XM, InvU<FR>>(su, PU, u, f, j0, nvalid, k);
+    load_full<T, K, 3, W + 1, XM, InvB<FR>>(sb, PB, b, f, j0, nvalid + 1, k);
+    load_comp<T, K, 3, W, XM>(sk, PK, b, Comp<FR>::b[2], f, j0, nvalid, kp);
+    __syncthreads();
+
+    const int w = threadIdx.x >> 6;
+    if (w < nvalid) {
+        const int l = lane_id();
+        const int lc = l % L;  // lanes >= L shadow lane l % L and store nothing
+        const int src_next = lc + 1 == L ? 0 : lc + 1, src_prev = lc == 0 ? L - 1 : lc - 1;
+        const T dt = *dtp;
+        T uu[K][5], bx[K][3];
+        const T *pu = su + w * PU + lc * (5 * K + 1);
+#pragma unroll
+        for (int q = 0; q < K; q++)
+#pragma unroll
+            for (int m = 0; m < 5; m++) uu[q][m] = pu[q * 5 + m];
+        // face B to cell centre (fluidx.c:71-81)
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            const int i = lc * K + q, ip = i + 1 == n ? 0 : i + 1;
+            const T b1 = sb[w * PB + pidx<K, 3>(i, 0)], b1p = sb[w * PB + pidx<K, 3>(ip, 0)];
+            const T b2 = sb[w * PB + pidx<K, 3>(i, 1)], b2p = sb[(w + 1) * PB + pidx<K, 3>(i, 1)];
+            const T b3 = sb[w * PB + pidx<K, 3>(i, 2)], b3p = sk[w * PK + pidx<K, 1>(i, 0)];
+            bx[q][0] = (T)0.5 * b1 + (T)0.5 * b1p;
+            bx[q][1] = (T)0.5 * b2 + b2p / (T)2;
+            bx[q][2] = (T)0.5 * b3 + b3p / (T)2;
+        }
+        tvd1_pencil<T, K>(uu, bx, dt, src_next, src_prev);
+        if (l < L) {
+            T *po = su + w * PU + lc * (5 * K + 1);
+#pragma unroll
+            for (int q = 0; q < K; q++)
+#pragma unroll
+                for (int m = 0; m < 5; m++) po[q * 5 + m] = uu[q][m];
+        }
+    }
+    __syncthreads();
+    store_full<T, K, 5, W, XM, InvU<FR>>(su, PU, u, f, j0, nvalid, k);
+}
+
+// ---------------------------------------------------------------------------
+// advectbyzx: Heterogeneous_MHD/C/advectbyzx.c:14-23 with A1/A2/B1/B2 fused.
+// Per tile the fluxes FA of pencils j0..j0+nvalid (neighbour j+1 needed by
+// b1), FB at k and at k+1 are computed (tvdb each), then
+// b1 = (((b1 - FA_j) + FA_{j+1}) - FB_k) + FB_{k+1}   (A1.c:55, A2.c:23, B1.c:54, B2.c:23).
+// ---------------------------------------------------------------------------
+template <typename T, int K, int W> __host__ __device__ constexpr int advect_lds_words(int n)
+{
+    // rho & momentum: (W+2) slots at k, W at k-1, W at k+1; b: W+1 slots (3 comps);
+    // b3 at k+1: W slots; fluxes FA (W+1), FB0 (W), FB1 (W)
+    return 2 * (3 * W + 2) * plen<K, 1>(n) + (W + 1) * plen<K, 3>(n) + W * plen<K, 1>(n) +
+           (3 * W + 1) * plen<K, 1>(n);
+}
+
+// tvdb on a pencil held K cells per lane (tvdb.c:10-91): bin -> bnew, and
+// F[i] = flux[i-1], the shifted flux the CT update uses.
+template <typename T, int K>
+__device__ __forceinline__ void tvdb_pencil(const T (&bin)[K], const T (&vg)[K], T dt, T (&bnew)[K], T (&F)[K],
+                                            int src_next, int src_prev)
+{
+    T vgn[K], vh[K], bv[K], bvn[K], f1[K], f1p[K], b1[K], g[K], gn[K], gnn[K], gp[K], fl[K], flp[K];
+    shift_next<T, K>(vg, vgn, src_next);
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+        vh[q] = (T)0.5 * vg[q] + (T)0.5 * vgn[q];
+        bv[q] = bin[q] * vg[q];
+    }
+    shift_next<T, K>(bv, bvn, src_next);
+#pragma unroll
+    for (int q = 0; q < K; q++) f1[q] = vh[q] > (T)0 ? bv[q] : bvn[q];
+    shift_prev<T, K>(f1, f1p, src_prev);
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+        b1[q] = (T)((double)bin[q] - (double)((f1[q] - f1p[q]) * dt) / 2.0);
+        g[q] = vg[q] * b1[q];
+    }
+    shift_next<T, K>(g, gn, src_next);
+    shift_next<T, K>(gn, gnn, src_next);
+    shift_prev<T, K>(g, gp, src_prev);
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+        T w, wp, wm;
+        if (vh[q] > (T)0) {
+            w = g[q];
+            wp = (T)((double)(gn[q] - w) / 2.0);
+            wm = (T)((double)(w - gp[q]) / 2.0);
+        } else {
+            w = gn[q];
+            wp = (T)((double)(w - gnn[q]) / 2.0);
+            wm = (T)((double)(g[q] - w) / 2.0);
+        }
+        const T dw = vanleer<T>(wm, wp);
+        fl[q] = (w + dw) * dt;
+    }
+    shift_prev<T, K>(fl, flp, src_prev);
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+        bnew[q] = bin[q] - (fl[q] - flp[q]);
+        F[q] = flp[q];
+    }
+}
+
+template <typename T, int K, int W, int FR>
+__global__ void __launch_bounds__(64 * W) advect_kernel(const T *__restrict__ u, T *__restrict__ b, Frame f,
+                                                        const T *__restrict__ dtp, int ntj)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *smem = reinterpret_cast<T *>(smem_raw);
+    constexpr bool XM = (FR == 0);
+    const int n = f.ni, L = n / K;
+    const int PK = plen<K, 1>(n), PB = plen<K, 3>(n);
+    // slot layout of the rho/momentum rows: 0..W+1 at k (pencil j0-1+s),
+    // W+2..2W+1 at k-1 (pencil j0+s), 2W+2..3W+1 at k+1 (pencil j0+s)
+    T *sr = smem, *sm = sr + (3 * W + 2) * PK;
+    T *sb = sm + (3 * W + 2) * PK, *sb3p = sb + (W + 1) * PB;
+    T *sF = sb3p + W * PK;  // FA: slots 0..W, FB0: W+1..2W, FB1: 2W+1..3W
+
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int k = tile / ntj, j0 = (tile - k * ntj) * W;
+    const int nvalid = min(W, f.nj - j0);
+    const int kp = (k + 1 == f.nk) ? 0 : k + 1, km = k == 0 ? f.nk - 1 : k - 1;
+    constexpr int pm = Comp<FR>::u[1];
+
+    load_comp<T, K, 5, W + 2, XM>(sr, PK, u, 0, f, j0 - 1 + f.nj, nvalid + 2, k);
+    load_comp<T, K, 5, W + 2, XM>(sm, PK, u, pm, f, j0 - 1 + f.nj, nvalid + 2, k);
+    load_comp<T, K, 5, W, XM>(sr + (W + 2) * PK, PK, u, 0, f, j0, nvalid, km);
+    load_comp<T, K, 5, W, XM>(sm + (W + 2) * PK, PK, u, pm, f, j0, nvalid, km);
+    load_comp<T, K, 5, W, XM>(sr + (2 * W + 2) * PK, PK, u, 0, f, j0, nvalid, kp);
+    load_comp<T, K, 5, W, XM>(sm + (2 * W + 2) * PK, PK, u, pm, f, j0, nvalid, kp);
+    load_full<T, K, 3, W + 1, XM, InvB<FR>>(sb, PB, b, f, j0, nvalid + 1, k);
+    load_comp<T, K, 3, W, XM>(sb3p, PK, b, Comp<FR>::b[2], f, j0, nvalid, kp);
+    __syncthreads();
+
+    const int w = threadIdx.x >> 6;
+    const int l = lane_id();
+    const int lc = l % L;
+    const int src_next = lc + 1 == L ? 0 : lc + 1, src_prev = lc == 0 ? L - 1 : lc - 1;
+    const T dt = *dtp;
+    const int nitem = 3 * nvalid + 1;
+    for (int it = w; it < nitem; it += W) {
+        // item -> (rows of the two velocity pencils, field source, outputs)
+        int ra, rb, fslot, fkind;  // fkind 0: b2 of sb slot, 1: b3 of sb slot, 2: b3 at k+1
+        int Fslot;
+        if (it <= nvalid) {  // FA(s): pencils j-1 (row s) and j (row s+1) at k, field b2
+            ra = it;
+            rb = it + 1;
+            fslot = it;
+            fkind = 0;
+            Fslot = it;
+        } else if (it <= 2 * nvalid) {  // FB0(s): pencil (s,k-1) and (s,k), field b3
+            const int s = it - nvalid - 1;
+            ra = W + 2 + s;
+            rb = s + 1;
+            fslot = s;
+            fkind = 1;
+            Fslot = W + 1 + s;
+        } else {  // FB1(s): pencil (s,k) and (s,k+1), field b3 at k+1
+            const int s = it - 2 * nvalid - 1;
+            ra = s + 1;
+            rb = 2 * W + 2 + s;
+            fslot = s;
+            fkind = 2;
+            Fslot = 2 * W + 1 + s;
+        }
+        T vx[K], vxm[K], vxn[K], vs[K], bin[K], bnew[K], F[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            const int o = pidx<K, 1>(lc * K + q, 0);
+            // A1.c:34 (u2(jm)+u2(j))/(u1(jm)+u1(j)); the minus-side pencil first
+            vx[q] = (sm[ra * PK + o] + sm[rb * PK + o]) / (sr[ra * PK + o] + sr[rb * PK + o]);
+        }
+        shift_prev<T, K>(vx, vxm, src_prev);
+        shift_next<T, K>(vx, vxn, src_next);
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            // A1.c:41 (vx[i-1] + vx[i+1] + 2.0*vx)/4.0
+            vs[q] = (T)(((double)(vxm[q] + vxn[q]) + 2.0 * (double)vx[q]) / 4.0);
+            const int i = lc * K + q;
+            if (fkind == 0)
+                bin[q] = sb[fslot * PB + pidx<K, 3>(i, 1)];
+            else if (fkind == 1)
+                bin[q] = sb[fslot * PB + pidx<K, 3>(i, 2)];
+            else
+                bin[q] = sb3p[fslot * PK + pidx<K, 1>(i, 0)];
+        }
+        tvdb_pencil<T, K>(bin, vs, dt, bnew, F, src_next, src_prev);
+        if (l < L) {
+#pragma unroll
+            for (int q = 0; q < K; q++) {
+                const int i = lc * K + q;
+                sF[Fslot * PK + pidx<K, 1>(i, 0)] = F[q];
+                // new b2 (FA) / b3 (FB0) of the tile's own pencils; the item only
+                // reads its own field row, so it is overwritten in place
+                if (fkind == 0 && fslot < nvalid) sb[fslot * PB + pidx<K, 3>(i, 1)] = bnew[q];
+                if (fkind == 1) sb[fslot * PB + pidx<K, 3>(i, 2)] = bnew[q];
+            }
+        }
+    }
+    __syncthreads();
+    if (w < nvalid && l < L) {
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            const int i = lc * K + q;
+            const int o = pidx<K, 1>(i, 0);
+            T b1 = sb[w * PB + pidx<K, 3>(i, 0)];
+            b1 = b1 - sF[w * PK + o];
+            b1 = b1 + sF[(w + 1) * PK + o];
+            b1 = b1 - sF[(W + 1 + w) * PK + o];
+            b1 = b1 + sF[(2 * W + 1 + w) * PK + o];
+            sb[w * PB + pidx<K, 3>(i, 0)] = b1;
+        }
+    }
+    __syncthreads();
+    store_full<T, K, 3, W, XM, InvB<FR>>(sb, PB, b, f, j0, nvalid, k);
+}
+
+// ---------------------------------------------------------------------------
+// cfl: Heterogeneous_MHD/C/calcfl.c:13-65 -> max over cells, as IEEE bits in
+// *cmax (all speeds are >= 0, so the unsigned order is the float order).
+// AVG selects the cell-averaged B^2 (OpenMP/mhd.f90:112-120).
+// ---------------------------------------------------------------------------
+template <typename T> __device__ __forceinline__ unsigned long long tobits(T x)
+{
+    if constexpr (sizeof(T) == 4)
+        return (unsigned long long)__float_as_uint(x);
+    else
+        return (unsigned long long)__double_as_longlong(x);
+}
+template <typename T> __device__ __forceinline__ T frombits(unsigned long long x)
+{
+    if constexpr (sizeof(T) == 4)
+        return __uint_as_float((unsigned)x);
+    else
+        return __longlong_as_double((long long)x);
+}
+
+template <typename T, bool AVG>
+__global__ void __launch_bounds__(256) cfl_kernel(const T *__restrict__ u, const T *__restrict__ b, int nx, int ny,
+                                                  int nz, unsigned long long *cmax)
+{
+    const long ncell = (long)nx * ny * nz;
+    T c = (T)0;
+    for (long s = (long)blockIdx.x * blockDim.x + threadIdx.x; s < ncell; s += (long)gridDim.x * blockDim.x) {
+        const T *pu = u + 5 * s;
+        const T *pb = b + 3 * s;
+        T B;
+        if constexpr (AVG) {
+            const int i = (int)(s % nx);
+            const long r = s / nx;
+            const int j = (int)(r % ny), k = (int)(r / ny);
+            const long sip = s - i + (i + 1 == nx ? 0 : i + 1);
+            const long sjp = ((long)k * ny + (j + 1 == ny ? 0 : j + 1)) * nx + i;
+            const long skp = ((long)(k + 1 == nz ? 0 : k + 1) * ny + j) * nx + i;
+            const T bx = (T)((double)(pb[0] + b[3 * sip + 0]) / 2.0);
+            const T by = (T)((double)(pb[1] + b[3 * sjp + 1]) / 2.0);
+            const T bz = (T)((double)(pb[2] + b[3 * skp + 2]) / 2.0);
+            B = bx * bx + by * by + bz * bz;
+        } else {
+            B = pb[0] * pb[0] + pb[1] * pb[1] + pb[2] * pb[2];
+        }
+        const T t = cfl_speed<T>(pu[0], pu[1], pu[2], pu[3], pu[4], B);
+        c = c > t ? c : t;  // max_2num(c, t)
+    }
+    c = wave_max(c);
+    __shared__ T red[4];
+    if (lane_id() == 0) red[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T m = red[0];
+        for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = fmax(m, red[i]);
+        atomicMax(cmax, tobits(m));
+    }
+}
+
+// dt update on the device (Heterogeneous_MHD/C/mhd.c:63-65): cfl = 1/c in
+// double rounded to T, dt = 0.9*cfl in double rounded to T, dt clamped to
+// (tf-t)/2, t += 2*dt (in T; an FP64 copy of t is accumulated beside it).
+// The max accumulator is cleared for the next step.
+template <typename T>
+__global__ void dt_kernel(unsigned long long *cmax, T *dt, T *t, double *t64, T tf, T *dts, int *iter, int cap)
+{
+    const T c = frombits<T>(*cmax);
+    *cmax = 0ull;
+    const T cfl = (T)(1.0 / (double)c);
+    T d = (T)(0.9 * (double)cfl);
+    if (d > (tf - *t) / (T)2) d = (tf - *t) / (T)2;
+    *t = *t + (T)2 * d;
+    *t64 = *t64 + 2.0 * (double)d;
+    *dt = d;
+    const int it = *iter;
+    if (it < cap) dts[it] = d;
+    *iter = it + 1;
+}
+
+// cfl value 1/c (calcfl.c:61) for the step-level entry point
+template <typename T> __global__ void cfl_value_kernel(unsigned long long *cmax, T *out)
+{
+    const T c = frombits<T>(*cmax);
+    *cmax = 0ull;
+    *out = (T)(1.0 / (double)c);
+}
+
+__global__ void zero_u64_kernel(unsigned long long *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// ---------------------------------------------------------------------------
+// Transposes of the step-level API (transpose12.c:7-47, transpose13.c:9-62):
+// out cell (b + a*sao + c3*sco) = perm(in cell (a + b*sbi + c3*sci)), a
+// contiguous in the input, b contiguous in the output.  32x32 cell tiles
+// through LDS, both sides coalesced.
+// ---------------------------------------------------------------------------
+template <typename T, int C>
+__global__ void __launch_bounds__(256) transpose_kernel(T *__restrict__ out, const T *__restrict__ in, int na, int nb,
+                                                        long sbi, long sci, long sao, long sco, int p0, int p1,
+                                                        int p2, int p3, int p4)
+{
+    __shared__ T tile[32][32 * C + 1];
+    const int perm[5] = {p0, p1, p2, p3, p4};
+    const int a0 = blockIdx.x * 32, b0 = blockIdx.y * 32, c3 = blockIdx.z;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < 32 * 32 * C; e += 256) {
+        const int bb = e / (32 * C), r = e - bb * (32 * C);
+        const int aa = r / C, p = r - aa * C;
+        if (a0 + aa < na && b0 + bb < nb)
+            tile[bb][aa * C + p] = in[((long)(a0 + aa) + (long)(b0 + bb) * sbi + (long)c3 * sci) * C + p];
+    }
+    __syncthreads();
+    for (int e = tid; e < 32 * 32 * C; e += 256) {
+        const int aa = e / (32 * C), r = e - aa * (32 * C);
+        const int bb = r / C, q = r - bb * C;
+        if (a0 + aa < na && b0 + bb < nb)
+            out[((long)(b0 + bb) + (long)(a0 + aa) * sao + (long)c3 * sco) * C + q] = tile[bb][aa * C + perm[q]];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 monitors: per-block partial sums (mass, energy, momentum x/y/z) and
+// max |div B| (left-face B), reduced in a fixed order -> deterministic.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) monitor_partial_kernel(const T *__restrict__ u, const T *__restrict__ b, int nx,
+                                                              int ny, int nz, double *part)
+{
+    const long ncell = (long)nx * ny * nz;
+    double a[6] = {0, 0, 0, 0, 0, 0};
+    for (long s = (long)blockIdx.x * blockDim.x + threadIdx.x; s < ncell; s += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(s % nx);
+        const long r = s / nx;
+        const int j = (int)(r % ny), k = (int)(r / ny);
+        const long sip = s - i + (i + 1 == nx ? 0 : i + 1);
+        const long sjp = ((long)k * ny + (j + 1 == ny ? 0 : j + 1)) * nx + i;
+        const long skp = ((long)(k + 1 == nz ? 0 : k + 1) * ny + j) * nx + i;
+        a[0] += (double)u[5 * s];
+        a[1] += (double)u[5 * s + 4];
+        a[2] += (double)u[5 * s + 1];
+        a[3] += (double)u[5 * s + 2];
+        a[4] += (double)u[5 * s + 3];
+        const double d = ((double)b[3 * sip] - (double)b[3 * s]) + ((double)b[3 * sjp + 1] - (double)b[3 * s + 1]) +
+                         ((double)b[3 * skp + 2] - (double)b[3 * s + 2]);
+        a[5] = fmax(a[5], fabs(d));
+    }
+    __shared__ double red[6][256];
+    for (int q = 0; q < 6; q++) red[q][threadIdx.x] = a[q];
+    __syncthreads();
+    for (int st = 128; st >= 1; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int q = 0; q < 6; q++)
+                red[q][threadIdx.x] = q == 5 ? fmax(red[q][threadIdx.x], red[q][threadIdx.x + st])
+                                             : red[q][threadIdx.x] + red[q][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void monitor_final_kernel(const double *part, int nblk, double *out)
+{
+    if (threadIdx.x < 6) {
+        const int q = threadIdx.x;
+        double a = 0;
+        for (int i = 0; i < nblk; i++) a = q == 5 ? fmax(a, part[i * 6 + q]) : a + part[i * 6 + q];
+        out[q] = a;
+    }
+}
+
+// ===========================================================================
+// host launchers
+// ===========================================================================
+int pick_k(int n)
+{
+    static const int ks[] = {1, 2, 3, 4, 5, 6, 8};
+    if (n < 2) return 0;
+    for (int K : ks)
+        if (K * 64 >= n && n % K == 0) return K;
+    return 0;
+}
+
+Frame make_frame(int fr, int nx, int ny, int nz)
+{
+    Frame f;
+    const long sx = 1, sy = nx, sz = (long)nx * ny;
+    if (fr == 0) {
+        f.ni = nx; f.nj = ny; f.nk = nz; f.si = sx; f.sj = sy; f.sk = sz;
+    } else if (fr == 1) {
+        f.ni = ny; f.nj = nx; f.nk = nz; f.si = sy; f.sj = sx; f.sk = sz;
+    } else {
+        f.ni = nz; f.nj = nx; f.nk = ny; f.si = sz; f.sj = sx; f.sk = sy;
+    }
+    return f;
+}
+
+template <typename T> constexpr int tile_w() { return sizeof(T) == 4 ? 4 : 2; }
+
+#define MHD_KSWITCH(K, CALL)                                                                                         \
+    switch (K) {                                                                                                    \
+    case 1: return CALL(1);                                                                                         \
+    case 2: return CALL(2);                                                                                         \
+    case 3: return CALL(3);                                                                                         \
+    case 4: return CALL(4);                                                                                         \
+    case 5: return CALL(5);                                                                                         \
+    case 6: return CALL(6);                                                                                         \
+    case 8: return CALL(8);                                                                                         \
+    default: return hipErrorInvalidValue;                                                                           \
+    }
+
+// raise the dynamic-LDS limit of a sweep kernel once per process (160 KB on gfx950)
+template <typename T, int K, int FR, int WHICH> static hipError_t sweep_attr()
+{
+    constexpr int W = tile_w<T>();
+    static const hipError_t e = WHICH == 0
+        ? hipFuncSetAttribute((const void *)fluidx_kernel<T, K, W, FR>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024)
+        : hipFuncSetAttribute((const void *)advect_kernel<T, K, W, FR>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+    return e;
+}
+
+template <typename T, int K, int FR> static hipError_t sweep_attr_both()
+{
+    hipError_t e = sweep_attr<T, K, FR, 0>();
+    return e != hipSuccess ? e : sweep_attr<T, K, FR, 1>();
+}
+
+template <typename T> hipError_t prepare_frame(int fr, int n)
+{
+    const int K = pick_k(n);
+#define PA0(KK) sweep_attr_both<T, KK, 0>()
+#define PA1(KK) sweep_attr_both<T, KK, 1>()
+#define PA2(KK) sweep_attr_both<T, KK, 2>()
+    if (fr == 0) { MHD_KSWITCH(K, PA0) }
+    if (fr == 1) { MHD_KSWITCH(K, PA1) }
+    MHD_KSWITCH(K, PA2)
+#undef PA0
+#undef PA1
+#undef PA2
+}
+
+template <typename T> hipError_t prepare_sweeps(int nx, int ny, int nz)
+{
+    hipError_t e = prepare_frame<T>(0, nx);
+    if (e == hipSuccess) e = prepare_frame<T>(1, ny);
+    if (e == hipSuccess) e = prepare_frame<T>(2, nz);
+    return e;
+}
+
+template <typename T, int K, int FR>
+static hipError_t fluidx_launch_k(T *u, const T *b, const Frame &f, const T *dt, hipStream_t st)
+{
+    constexpr int W = tile_w<T>();
+    const size_t lds = sizeof(T) * fluidx_lds_words<T, K, W>(f.ni);
+    auto kern = fluidx_kernel<T, K, W, FR>;
+    const hipError_t ea = sweep_attr<T, K, FR, 0>();
+    if (ea != hipSuccess) return ea;
+    const int ntj = (f.nj + W - 1) / W;
+    const int nblk = ntj * f.nk;
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(64 * W), lds, st, u, b, f, dt, ntj);
+    return hipGetLastError();
+}
+
+template <typename T, int K, int FR>
+static hipError_t advect_launch_k(const T *u, T *b, const Frame &f, const T *dt, hipStream_t st)
+{
+    constexpr int W = tile_w<T>();
+    const size_t lds = sizeof(T) * advect_lds_words<T, K, W>(f.ni);
+    auto kern = advect_kernel<T, K, W, FR>;
+    const hipError_t ea = sweep_attr<T, K, FR, 1>();
+    if (ea != hipSuccess) return ea;
+    const int ntj = (f.nj + W - 1) / W;
+    const int nblk = ntj * f.nk;
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(64 * W), lds, st, u, b, f, dt, ntj);
+    return hipGetLastError();
+}
+
+
+template <typename T>
+hipError_t launch_fluidx(T *u, const T *b, int fr, int nx, int ny, int nz, const T *dt, hipStream_t st)
+{
+    const Frame f = make_frame(fr, nx, ny, nz);
+    const int K = pick_k(f.ni);
+#define FX0(KK) fluidx_launch_k<T, KK, 0>(u, b, f, dt, st)
+#define FX1(KK) fluidx_launch_k<T, KK, 1>(u, b, f, dt, st)
+#define FX2(KK) fluidx_launch_k<T, KK, 2>(u, b, f, dt, st)
+    if (fr == 0) { MHD_KSWITCH(K, FX0) }
+    if (fr == 1) { MHD_KSWITCH(K, FX1) }
+    MHD_KSWITCH(K, FX2)
+#undef FX0
+#undef FX1
+#undef FX2
+}
+
+template <typename T>
+hipError_t launch_advect(const T *u, T *b, int fr, int nx, int ny, int nz, const T *dt, hipStream_t st)
+{
+    const Frame f = make_frame(fr, nx, ny, nz);
+    const int K = pick_k(f.ni);
+#define AX0(KK) advect_launch_k<T, KK, 0>(u, b, f, dt, st)
+#define AX1(KK) advect_launch_k<T, KK, 1>(u, b, f, dt, st)
+#define AX2(KK) advect_launch_k<T, KK, 2>(u, b, f, dt, st)
+    if (fr == 0) { MHD_KSWITCH(K, AX0) }
+    if (fr == 1) { MHD_KSWITCH(K, AX1) }
+    MHD_KSWITCH(K, AX2)
+#undef AX0
+#undef AX1
+#undef AX2
+}
+
+static int cfl_blocks(long ncell)
+{
+    long nb = (ncell + 255) / 256;
+    return (int)(nb < 4096 ? nb : 4096);
+}
+
+template <typename T>
+hipError_t launch_cfl(const T *u, const T *b, int nx, int ny, int nz, int avg, unsigned long long *cmax,
+                      hipStream_t st)
+{
+    const int nb = cfl_blocks((long)nx * ny * nz);
+    if (avg)
+        hipLaunchKernelGGL((cfl_kernel<T, true>), dim3(nb), dim3(256), 0, st, u, b, nx, ny, nz, cmax);
+    else
+        hipLaunchKernelGGL((cfl_kernel<T, false>), dim3(nb), dim3(256), 0, st, u, b, nx, ny, nz, cmax);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_dt(unsigned long long *cmax, T *dt, T *t, double *t64, T tf, T *dts, int *iter, int cap,
+                     hipStream_t st)
+{
+    hipLaunchKernelGGL(dt_kernel<T>, dim3(1), dim3(1), 0, st, cmax, dt, t, t64, tf, dts, iter, cap);
+    return hipGetLastError();
+}
+
+template <typename T> hipError_t launch_cfl_value(unsigned long long *cmax, T *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(cfl_value_kernel<T>, dim3(1), dim3(1), 0, st, cmax, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_u64(unsigned long long *p, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(zero_u64_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_transpose(T *uo, T *bo, const T *u, const T *b, int which, int nx, int ny, int nz, hipStream_t st)
+{
+    // which 12: (i,j,k)->(j,i,k), u (1,3,2,4,5), b (2,1,3)
+    // which 13: (i,j,k)->(k,j,i), u (1,4,3,2,5), b (3,2,1)
+    int na = nx, nb, nc;
+    long sbi, sci, sao, sco;
+    int pu[5], pb[3];
+    if (which == 12) {
+        nb = ny; nc = nz; sbi = nx; sci = (long)nx * ny; sao = ny; sco = (long)nx * ny;
+        const int a[5] = {0, 2, 1, 3, 4}, c[3] = {1, 0, 2};
+        for (int i = 0; i < 5; i++) pu[i] = a[i];
+        for (int i = 0; i < 3; i++) pb[i] = c[i];
+    } else {
+        nb = nz; nc = ny; sbi = (long)nx * ny; sci = nx; sao = (long)ny * nz; sco = nz;
+        const int a[5] = {0, 3, 2, 1, 4}, c[3] = {2, 1, 0};
+        for (int i = 0; i < 5; i++) pu[i] = a[i];
+        for (int i = 0; i < 3; i++) pb[i] = c[i];
+    }
+    dim3 grid((na + 31) / 32, (nb + 31) / 32, nc);
+    hipLaunchKernelGGL((transpose_kernel<T, 5>), grid, dim3(256), 0, st, uo, u, na, nb, sbi, sci, sao, sco, pu[0],
+                       pu[1], pu[2], pu[3], pu[4]);
+    hipLaunchKernelGGL((transpose_kernel<T, 3>), grid, dim3(256), 0, st, bo, b, na, nb, sbi, sci, sao, sco, pb[0],
+                       pb[1], pb[2], 0, 0);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_monitors(const T *u, const T *b, int nx, int ny, int nz, double *part, int nblk, double *out,
+                           hipStream_t st)
+{
+    hipLaunchKernelGGL(monitor_partial_kernel<T>, dim3(nblk), dim3(256), 0, st, u, b, nx, ny, nz, part);
+    hipLaunchKernelGGL(monitor_final_kernel, dim3(1), dim3(64), 0, st, part, nblk, out);
+    return hipGetLastError();
+}
+
+#define MHD_INST(T)                                                                                                  \
+    template hipError_t launch_fluidx<T>(T *, const T *, int, int, int, int, const T *, hipStream_t);              \
+    template hipError_t launch_advect<T>(const T *, T *, int, int, int, int, const T *, hipStream_t);              \
+    template hipError_t launch_cfl<T>(const T *, const T *, int, int, int, int, unsigned long long *, hipStream_t); \
+    template hipError_t launch_dt<T>(unsigned long long *, T *, T *, double *, T, T *, int *, int, hipStream_t);    \
+    template hipError_t launch_cfl_value<T>(unsigned long long *, T *, hipStream_t);                               \
+    template hipError_t launch_transpose<T>(T *, T *, const T *, const T *, int, int, int, int, hipStream_t);       \
+    template hipError_t launch_monitors<T>(const T *, const T *, int, int, int, double *, int, double *, hipStream_t); \
+    template hipError_t prepare_sweeps<T>(int, int, int);
+MHD_INST(float)
+MHD_INST(double)
+
+}  // namespace mhd

@@ -1,0 +1,275 @@
+"""Python host mirror of the MI355X relaxing-TVD MHD library (libmhdgpu.so).
+
+The functions keep the reference's operator interface for the timestep
+(HIP/hip_subroutine.h:4-12, Heterogeneous_MHD/C/spu_function.h:1-15):
+``fluidx``, ``advectbyzx``, ``transpose12``, ``transpose13``, ``cfl`` and
+``init``/``main``, over AoS ``u`` (5 per cell) and ``b`` (3 per cell) arrays,
+with the same argument meaning.  Errors raise ``MHDError`` carrying the
+library's code and message (the reference prints and continues or exits).
+
+Everything runs through the C ABI of include/mhd_api.h with ctypes; device
+memory is held in ``DeviceArray`` objects.  There is no CPU fallback: if the
+library is missing, importing works but the first call raises.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("MHDGPU_LIB", os.path.join(os.path.dirname(_HERE), "libmhdgpu.so"))
+
+MHD_OK = 0
+ERRORS = {-1: "MHD_ERR_ARG", -2: "MHD_ERR_SHAPE", -3: "MHD_ERR_HIP", -4: "MHD_ERR_NOMEM", -5: "MHD_ERR_STATE",
+          -6: "MHD_ERR_IO", -7: "MHD_ERR_FULL", -8: "MHD_ERR_NONFINITE"}
+
+# every C entry point of include/mhd_api.h: name -> (restype, argtypes)
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_IP = ctypes.POINTER(ctypes.c_int)
+_FP = ctypes.POINTER(ctypes.c_float)
+_DP = ctypes.POINTER(ctypes.c_double)
+_SZ = ctypes.c_size_t
+_SZP = ctypes.POINTER(ctypes.c_size_t)
+SIGNATURES = {
+    "mhd_last_error": (ctypes.c_char_p, []),
+    "mhd_version": (ctypes.c_char_p, []),
+    "mhd_dev_alloc": (_I, [ctypes.POINTER(_P), _SZ]),
+    "mhd_dev_free": (_I, [_P]),
+    "mhd_memcpy_h2d": (_I, [_P, _P, _SZ]),
+    "mhd_memcpy_d2h": (_I, [_P, _P, _SZ]),
+    "mhd_memcpy_d2d": (_I, [_P, _P, _SZ]),
+    "mhd_device_sync": (_I, []),
+    "mhd_mem_info": (_I, [_SZP, _SZP]),
+    "mhd_fluidx": (_I, [_P, _P, _IP, _IP, _IP, _P]),
+    "mhd_advectbyzx": (_I, [_P, _P, _IP, _IP, _IP, _P]),
+    "mhd_transpose12": (_I, [_P, _P, _P, _P, _IP, _IP, _IP]),
+    "mhd_transpose13": (_I, [_P, _P, _P, _P, _IP, _IP, _IP]),
+    "mhd_cfl": (_I, [_P, _P, _IP, _IP, _IP, _I, _P, _FP]),
+    "mhd_init": (_I, [ctypes.c_char_p, _I, _P, _P, _IP, _IP, _IP]),
+    "mhd_write_result": (_I, [ctypes.c_char_p, ctypes.c_char_p, _P, _P, _IP, _IP, _IP]),
+    "mhd_main": (_I, [_P, _P, _IP, _IP, _IP, _I, ctypes.c_float, _FP, _I, _IP]),
+    "mhd_ctx_create": (_I, [ctypes.POINTER(_P), _I, _I, _I, _I, _I]),
+    "mhd_ctx_destroy": (_I, [_P]),
+    "mhd_ctx_set_state": (_I, [_P, _P, _P]),
+    "mhd_ctx_get_state": (_I, [_P, _P, _P]),
+    "mhd_ctx_reset_clock": (_I, [_P, ctypes.c_double]),
+    "mhd_ctx_step": (_I, [_P, _I]),
+    "mhd_ctx_step_timed": (_I, [_P, _I, _FP]),
+    "mhd_ctx_sync": (_I, [_P]),
+    "mhd_ctx_dts": (_I, [_P, _DP, _I, _IP]),
+    "mhd_ctx_time": (_I, [_P, _DP, _DP]),
+    "mhd_ctx_monitors": (_I, [_P, _DP]),
+    "mhd_ctx_device_state": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
+    "mhd_ctx_profile_step": (_I, [_P, _I, _FP]),
+}
+
+
+class MHDError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libmhdgpu.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MHDError(-3, f"{LIB_PATH} not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != MHD_OK:
+        raise MHDError(rc, lib().mhd_last_error().decode(errors="replace"))
+    return rc
+
+
+def _iref(v):
+    return ctypes.byref(ctypes.c_int(int(v)))
+
+
+class DeviceArray:
+    """A device allocation made by the library, with host copies in and out."""
+
+    def __init__(self, nbytes, dtype=np.float32):
+        self.nbytes = int(nbytes)
+        self.dtype = np.dtype(dtype)
+        p = ctypes.c_void_p()
+        check(lib().mhd_dev_alloc(ctypes.byref(p), self.nbytes))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes, a.dtype)
+        check(lib().mhd_memcpy_h2d(d.ptr, a.ctypes.data, a.nbytes))
+        return d
+
+    def to_host(self, count=None):
+        n = self.nbytes // self.dtype.itemsize if count is None else count
+        out = np.empty(n, self.dtype)
+        check(lib().mhd_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def copy_from(self, a):
+        a = np.ascontiguousarray(a, self.dtype)
+        check(lib().mhd_memcpy_h2d(self.ptr, a.ctypes.data, a.nbytes))
+
+    def free(self):
+        if self.ptr:
+            lib().mhd_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ---- step-level operators (reference interface; device arrays, FP32) -------
+
+def fluidx(u, b, nx, ny, nz, dt):
+    """One relaxing-TVD sweep along i (Heterogeneous_MHD/C/fluidx.c:19); dt is a device scalar."""
+    check(lib().mhd_fluidx(u.ptr, b.ptr, _iref(nx), _iref(ny), _iref(nz), dt.ptr))
+
+
+def advectbyzx(u, b, nx, ny, nz, dt):
+    """CT advection of b2, b3 along i (Heterogeneous_MHD/C/advectbyzx.c:14)."""
+    check(lib().mhd_advectbyzx(u.ptr, b.ptr, _iref(nx), _iref(ny), _iref(nz), dt.ptr))
+
+
+def transpose12(ut, bt, u, b, nx, ny, nz):
+    """(i,j,k)->(j,i,k); afterwards u, b hold the (ny,nx,nz) frame (transpose12.c:7)."""
+    check(lib().mhd_transpose12(ut.ptr, bt.ptr, u.ptr, b.ptr, _iref(nx), _iref(ny), _iref(nz)))
+
+
+def transpose13(ut, bt, u, b, nx, ny, nz):
+    """(i,j,k)->(k,j,i); afterwards u, b hold the (nz,ny,nx) frame (transpose13.c:9)."""
+    check(lib().mhd_transpose13(ut.ptr, bt.ptr, u.ptr, b.ptr, _iref(nx), _iref(ny), _iref(nz)))
+
+
+def cfl(u, b, nx, ny, nz, d_cfl, avg_b=False):
+    """1/max fast speed (calcfl.c:13); written to the device scalar d_cfl, returned as float."""
+    h = ctypes.c_float()
+    check(lib().mhd_cfl(u.ptr, b.ptr, _iref(nx), _iref(ny), _iref(nz), int(avg_b), d_cfl.ptr, ctypes.byref(h)))
+    return h.value
+
+
+def init(path, n, order=0):
+    """Read an initial-state file (8 float32 per cell) into host (u, b) (init.c:11)."""
+    nc = n ** 3
+    u = np.empty(5 * nc, np.float32)
+    b = np.empty(3 * nc, np.float32)
+    check(lib().mhd_init(str(path).encode(), int(order), u.ctypes.data, b.ctypes.data, _iref(n), _iref(n),
+                         _iref(n)))
+    return u, b
+
+
+def write_result(path_u, path_b, u, b, nx, ny, nz):
+    u = np.ascontiguousarray(u, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    check(lib().mhd_write_result(str(path_u).encode(), str(path_b).encode(), u.ctypes.data, b.ctypes.data,
+                                 _iref(nx), _iref(ny), _iref(nz)))
+
+
+def main(u, b, nx, ny, nz, nsteps, tf=math.inf):
+    """Program-level loop (cuda_main, HIP/cu/cuda_main.cu:14) on host arrays, in place.
+    Returns the dt sequence."""
+    cap = max(int(nsteps), 1) if nsteps > 0 else 1 << 16
+    dts = np.zeros(cap, np.float32)
+    done = ctypes.c_int()
+    check(lib().mhd_main(u.ctypes.data, b.ctypes.data, _iref(nx), _iref(ny), _iref(nz), int(nsteps),
+                         ctypes.c_float(tf), dts.ctypes.data_as(_FP), cap, ctypes.byref(done)))
+    return dts[:min(done.value, cap)]
+
+
+def mem_info():
+    f, t = ctypes.c_size_t(), ctypes.c_size_t()
+    check(lib().mhd_mem_info(ctypes.byref(f), ctypes.byref(t)))
+    return f.value, t.value
+
+
+class Simulation:
+    """A device-resident box advanced by the library's captured timestep."""
+
+    def __init__(self, nx, ny, nz, precision=4, avg_b=False):
+        self.shape = (nx, ny, nz)
+        self.precision = precision
+        self.dtype = np.float32 if precision == 4 else np.float64
+        p = ctypes.c_void_p()
+        check(lib().mhd_ctx_create(ctypes.byref(p), nx, ny, nz, precision, int(avg_b)))
+        self.ctx = p.value
+        self.ncell = nx * ny * nz
+
+    def set_state(self, u, b):
+        u = np.ascontiguousarray(u, self.dtype)
+        b = np.ascontiguousarray(b, self.dtype)
+        assert u.size == 5 * self.ncell and b.size == 3 * self.ncell
+        check(lib().mhd_ctx_set_state(self.ctx, u.ctypes.data, b.ctypes.data))
+
+    def get_state(self):
+        u = np.empty(5 * self.ncell, self.dtype)
+        b = np.empty(3 * self.ncell, self.dtype)
+        check(lib().mhd_ctx_get_state(self.ctx, u.ctypes.data, b.ctypes.data))
+        return u, b
+
+    def reset_clock(self, tf=math.inf):
+        check(lib().mhd_ctx_reset_clock(self.ctx, float(tf)))
+
+    def step(self, nsteps=1):
+        check(lib().mhd_ctx_step(self.ctx, int(nsteps)))
+
+    def step_timed(self, nsteps):
+        ms = ctypes.c_float()
+        check(lib().mhd_ctx_step_timed(self.ctx, int(nsteps), ctypes.byref(ms)))
+        return ms.value
+
+    def sync(self):
+        check(lib().mhd_ctx_sync(self.ctx))
+
+    def dts(self, maxn=1 << 20):
+        cnt = ctypes.c_int()
+        out = np.zeros(maxn, np.float64)
+        check(lib().mhd_ctx_dts(self.ctx, out.ctypes.data_as(_DP), maxn, ctypes.byref(cnt)))
+        return out[:min(cnt.value, maxn)].astype(self.dtype)
+
+    def time(self):
+        t, t64 = ctypes.c_double(), ctypes.c_double()
+        check(lib().mhd_ctx_time(self.ctx, ctypes.byref(t), ctypes.byref(t64)))
+        return t.value, t64.value
+
+    def monitors(self):
+        out = np.zeros(6, np.float64)
+        check(lib().mhd_ctx_monitors(self.ctx, out.ctypes.data_as(_DP)))
+        return out
+
+    def profile_step(self, reps=3):
+        out = (ctypes.c_float * 13)()
+        check(lib().mhd_ctx_profile_step(self.ctx, int(reps), out))
+        return np.array(out[:], np.float64)
+
+    def close(self):
+        if self.ctx:
+            lib().mhd_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,233 @@
+"""GPU parity: every op of the timestep through the C ABI (libmhdgpu.so)
+against the oracle on the same seeded inputs.
+
+The bar is the reference's own results: every value equal to the oracle's
+(which equals the reference C path bit for bit), the sign of exact zeros
+aside, and the dt sequence bit-exact.  Pencil lengths cover every cells-per-
+lane variant (K = 1..8) and partial waves (L < 64 lanes).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, bitwise, same
+from golden.states import random_state
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+
+
+@pytest.fixture(scope="module")
+def M():
+    import mhdgpu
+    free, total = mhdgpu.mem_info()  # raises if the library or the GPU is missing
+    assert total > 0
+    return mhdgpu
+
+
+def dscalar(M, x, dtype=F32):
+    return M.DeviceArray.from_host(np.array([x], dtype))
+
+
+def gpu_op(M, op, u0, b0, nx, ny, nz, dt):
+    du, db = M.DeviceArray.from_host(u0), M.DeviceArray.from_host(b0)
+    getattr(M, op)(du, db, nx, ny, nz, dscalar(M, dt))
+    return du.to_host(), db.to_host()
+
+
+SHAPES = [(16, 4, 3), (48, 5, 4), (64, 3, 3), (96, 3, 5), (128, 2, 3), (192, 2, 2), (256, 3, 2), (320, 1, 2),
+          (384, 2, 1), (512, 1, 2), (24, 1, 1), (40, 7, 1)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_fluidx_matches_oracle(M, oracle, shape):
+    nx, ny, nz = shape
+    print("fluidx", shape, flush=True)
+    u0, b0 = random_state(nx, ny, nz, seed=100 + nx, amp=0.4)
+    dt = F32(0.23)
+    ug, bg = gpu_op(M, "fluidx", u0, b0, nx, ny, nz, dt)
+    uo, bo = u0.copy(), b0.copy()
+    oracle.fluidx(uo, bo, nx, ny, nz, dt)
+    assert same(ug, uo)
+    assert bitwise(bg, b0)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_advectbyzx_matches_oracle(M, oracle, shape):
+    nx, ny, nz = shape
+    print("advectbyzx", shape, flush=True)
+    u0, b0 = random_state(nx, ny, nz, seed=200 + nx, amp=0.5)
+    dt = F32(0.31)
+    ug, bg = gpu_op(M, "advectbyzx", u0, b0, nx, ny, nz, dt)
+    uo, bo = u0.copy(), b0.copy()
+    oracle.advectbyzx(uo, bo, nx, ny, nz, dt)
+    assert bitwise(ug, u0)
+    assert same(bg, bo)
+
+
+def test_single_ops_on_reference_fixtures(M, oracle, golden, fixture16, fixture32):
+    u0, b0 = fixture16
+    ug, _ = gpu_op(M, "fluidx", u0, b0, 16, 16, 16, F32(0.3))
+    assert same(ug, golden["fix16_fluidx_u"])
+    _, bg = gpu_op(M, "advectbyzx", u0, b0, 16, 16, 16, F32(0.3))
+    assert same(bg, golden["fix16_advect_b"])
+    u0, b0 = fixture32
+    ug, _ = gpu_op(M, "fluidx", u0, b0, 32, 32, 32, F32(0.41))
+    uo = u0.copy()
+    oracle.fluidx(uo, b0.copy(), 32, 32, 32, F32(0.41))
+    assert same(ug, uo)
+
+
+@pytest.mark.parametrize("shape", [(6, 5, 4), (32, 32, 32), (33, 17, 9), (40, 3, 70)])
+def test_transposes_match_oracle(M, oracle, shape):
+    nx, ny, nz = shape
+    u0, b0 = random_state(nx, ny, nz, seed=300)
+    for which in (12, 13):
+        du, db = M.DeviceArray.from_host(u0), M.DeviceArray.from_host(b0)
+        ut, bt = M.DeviceArray(u0.nbytes), M.DeviceArray(b0.nbytes)
+        getattr(M, f"transpose{which}")(ut, bt, du, db, nx, ny, nz)
+        uo, bo = getattr(oracle, f"transpose{which}")(u0, b0, nx, ny, nz)
+        assert bitwise(du.to_host(), uo) and bitwise(db.to_host(), bo)
+        assert bitwise(ut.to_host(), uo)
+
+
+@pytest.mark.parametrize("shape,avg", [((16, 16, 16), False), ((33, 8, 5), False), ((24, 12, 6), True),
+                                       ((128, 64, 8), False)])
+def test_cfl_matches_oracle(M, oracle, shape, avg):
+    nx, ny, nz = shape
+    u0, b0 = random_state(nx, ny, nz, seed=400, amp=0.5)
+    d = dscalar(M, 0.0)
+    c = M.cfl(M.DeviceArray.from_host(u0), M.DeviceArray.from_host(b0), nx, ny, nz, d, avg_b=avg)
+    assert F32(c) == F32(oracle.calcfl(u0, b0, nx, ny, nz, avg_b=avg))
+    assert d.to_host()[0] == F32(c)
+
+
+def test_ctx_ten_steps_equal_reference_golden(M, golden, fixture16):
+    sim = M.Simulation(16, 16, 16)
+    sim.set_state(*fixture16)
+    sim.step(10)
+    u, b = sim.get_state()
+    assert bitwise(sim.dts(), golden["fix16_run10_dt"])
+    assert same(u, golden["fix16_run10_u"]) and same(b, golden["fix16_run10_b"])
+    sim.close()
+
+
+def test_config1_alfven_32cube_200_steps(M, oracle):
+    """The CPU-runnable workload: 32^3 linear Alfven wave, 200 steps, FP32."""
+    from mhdgpu.ics import alfven_linear
+    u0, b0 = alfven_linear(32)
+    sim = M.Simulation(32, 32, 32)
+    sim.set_state(u0, b0)
+    sim.step(200)
+    ug, bg = sim.get_state()
+    uo, bo = u0.copy(), b0.copy()
+    do = oracle.run(uo, bo, 32, 32, 32, 200)
+    assert bitwise(sim.dts(), do)
+    assert same(ug, uo) and same(bg, bo)
+    rel = np.linalg.norm(ug.astype(np.float64) - uo) / np.linalg.norm(uo)
+    assert rel <= 1e-5
+    sim.close()
+
+
+@pytest.mark.parametrize("shape,seed,amp,steps,tf", [((24, 16, 32), 501, 0.4, 5, math.inf),
+                                                     ((16, 16, 16), 502, 0.55, 6, 3.0),
+                                                     ((48, 96, 64), 503, 0.3, 2, math.inf)])
+def test_ctx_steps_match_oracle(M, oracle, shape, seed, amp, steps, tf):
+    nx, ny, nz = shape
+    u0, b0 = random_state(nx, ny, nz, seed=seed, amp=amp)
+    sim = M.Simulation(nx, ny, nz)
+    sim.set_state(u0, b0)
+    sim.reset_clock(tf)
+    sim.step(steps)
+    ug, bg = sim.get_state()
+    uo, bo = u0.copy(), b0.copy()
+    do = oracle.run(uo, bo, nx, ny, nz, steps, tf=F32(tf))
+    assert bitwise(sim.dts(), do)
+    assert same(ug, uo) and same(bg, bo)
+    t, t64 = sim.time()
+    assert abs(t64 - 2 * do.astype(np.float64).sum()) < 1e-12
+    sim.close()
+
+
+def test_ctx_fp64_matches_fp64_oracle(M, oracle):
+    n = 12
+    u0, b0 = random_state(n, n, n, seed=601, amp=0.3)
+    sim = M.Simulation(n, n, n, precision=8)
+    sim.set_state(u0.astype(np.float64), b0.astype(np.float64))
+    sim.step(3)
+    ug, bg = sim.get_state()
+    uo, bo = u0.astype(np.float64), b0.astype(np.float64)
+    do = oracle.run(uo, bo, n, n, n, 3)
+    assert bitwise(sim.dts(), do)
+    assert same(ug, uo) and same(bg, bo)
+    sim.close()
+
+
+def test_avg_b_cfl_variant_in_driver(M, oracle):
+    n = 16
+    u0, b0 = random_state(n, n, n, seed=602, amp=0.4)
+    sim = M.Simulation(n, n, n, avg_b=True)
+    sim.set_state(u0, b0)
+    sim.step(3)
+    uo, bo = u0.copy(), b0.copy()
+    do = oracle.run(uo, bo, n, n, n, 3, avg_b=True)
+    assert bitwise(sim.dts(), do)
+    ug, bg = sim.get_state()
+    assert same(ug, uo) and same(bg, bo)
+    sim.close()
+
+
+def test_main_drop_in(M, oracle, fixture16):
+    u, b = (a.copy() for a in fixture16)
+    dts = M.main(u, b, 16, 16, 16, 4)
+    uo, bo = (a.copy() for a in fixture16)
+    do = oracle.run(uo, bo, 16, 16, 16, 4)
+    assert bitwise(dts, do) and same(u, uo) and same(b, bo)
+    # run-to-tf mode of the reference loop (t < tf)
+    u, b = (a.copy() for a in fixture16)
+    dts = M.main(u, b, 16, 16, 16, 0, tf=5.0)
+    uo, bo = (a.copy() for a in fixture16)
+    do = oracle.run(uo, bo, 16, 16, 16, len(dts), tf=F32(5.0))
+    assert bitwise(dts, do) and same(u, uo)
+
+
+def test_monitors_and_determinism(M, oracle):
+    n = 32
+    u0, b0 = random_state(n, n, n, seed=701, amp=0.3)
+    outs = []
+    for _ in range(2):
+        sim = M.Simulation(n, n, n)
+        sim.set_state(u0, b0)
+        m0 = sim.monitors()
+        sim.step(3)
+        outs.append(sim.get_state())
+        m1 = sim.monitors()
+        sim.close()
+    assert bitwise(outs[0][0], outs[1][0]) and bitwise(outs[0][1], outs[1][1])
+    mo = oracle.monitors(u0, b0, n, n, n)
+    assert np.allclose(m0, mo, rtol=1e-12, atol=1e-12)
+    assert abs(m1[0] - m0[0]) <= 1e-6 * abs(m0[0])  # mass
+    assert abs(m1[1] - m0[1]) <= 1e-5 * abs(m0[1])  # energy
+    assert abs(m1[5] - m0[5]) < 1e-5  # the CT update keeps div B
+
+
+def test_full_size_256_step_properties_and_parity(M, oracle):
+    """The 256^3 workload (random-phase velocity, curl-A field): one step on
+    the GPU against the oracle, plus the size-independent checks."""
+    from mhdgpu.ics import random_phase
+    n = 256
+    u0, b0 = random_phase(n, seed=2)
+    sim = M.Simulation(n, n, n)
+    sim.set_state(u0, b0)
+    m0 = sim.monitors()
+    sim.step(1)
+    ug, bg = sim.get_state()
+    m1 = sim.monitors()
+    sim.close()
+    assert m0[5] < 1e-5 and m1[5] < 1e-5  # div B from curl A stays at rounding level
+    assert abs(m1[0] - m0[0]) <= 1e-6 * m0[0]
+    uo, bo = u0.copy(), b0.copy()
+    do = oracle.run(uo, bo, n, n, n, 1)
+    assert same(ug, uo) and same(bg, bo)
