@@ -1,0 +1,152 @@
+"""Benchmark of the relaxing-TVD MHD timestep on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Workload (the headline configuration): a 256^3 periodic FP32 box, the
+Alfven background (rho0 = 1, p0 = 0.6, B0 = x) plus a seeded random-phase
+velocity field (P(k) ~ k^-2, k <= 8, rms Mach 0.1) and a curl-A magnetic
+perturbation (rms 0.1 B0).  One step = one loop iteration of the reference
+(1 CFL reduction + dt + 12 sweeps, advancing t by 2 dt).  The state is
+resident in HBM before the timed region; the K timed steps are replays of
+the captured step graph, bracketed by a device synchronisation on both
+sides.  Prints ONE JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "mhd-gpu_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+FLUIDX_BYTES_PER_CELL = 52   # u read 20 + b read 12 + u write 20
+ADVECT_BYTES_PER_CELL = 32   # u1,u2 read 8 + b read 12 + b write 12
+CFL_BYTES_PER_CELL = 32      # u, b read
+STEP_BYTES_PER_CELL = 6 * FLUIDX_BYTES_PER_CELL + 6 * ADVECT_BYTES_PER_CELL + CFL_BYTES_PER_CELL  # 536
+SWEEPS = ["fluidx_x", "advect_x", "fluidx_y", "advect_y", "fluidx_z", "advect_z", "advect_z", "fluidx_z",
+          "advect_y", "fluidx_y", "advect_x", "fluidx_x"]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pmc_traffic(kernel="fluidx", n=256):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{n}cube_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def cpu_baseline(u0, b0, n, seconds_hint=30.0):
+    """The reference's own C path (oracle/_ref, serial, as the reference runs)
+    on one step of the same workload, or the oracle restatement if the
+    reference build is absent."""
+    import oracle_lib
+    u, b = u0.copy(), b0.copy()
+    if oracle_lib.reference_available():
+        ref = oracle_lib.Reference()
+        kind, cores = "reference", 1
+        t0 = time.perf_counter()
+        ref.run(u, b, n, n, n, 1)
+        dt = time.perf_counter() - t0
+    else:
+        orc = oracle_lib.Oracle()
+        kind, cores = "port", os.cpu_count() or 1
+        t0 = time.perf_counter()
+        orc.run(u, b, n, n, n, 1)
+        dt = time.perf_counter() - t0
+    return {"value": n ** 3 / dt, "unit": "zone-updates/s", "cores": cores, "kind": kind,
+            "sample": f"1 step of the {n}^3 workload (same initial state), {dt:.1f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it", file=sys.stderr)
+        sys.exit(2)
+
+    import mhdgpu
+    from mhdgpu.ics import random_phase
+
+    n = args.n
+    log(f"bench: generating {n}^3 initial state")
+    u0, b0 = random_phase(n, seed=2)
+    sim = mhdgpu.Simulation(n, n, n)
+    sim.set_state(u0, b0)
+    log("bench: warmup")
+    sim.step(args.warmup)
+    sim.sync()
+    mhdgpu.check(mhdgpu.lib().mhd_device_sync())
+    t0 = time.perf_counter()
+    ev_ms = sim.step_timed(args.steps)
+    mhdgpu.check(mhdgpu.lib().mhd_device_sync())
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    zus = n ** 3 * args.steps / wall
+    log(f"bench: {args.steps} steps in {wall * 1e3:.1f} ms (events {ev_ms:.1f} ms) -> {zus / 1e9:.3f} G zone-updates/s")
+
+    # per-kernel times with HIP events on the context's stream
+    prof = sim.profile_step(reps=5)
+    sweep_ms = prof[1:]
+    fl = [sweep_ms[i] for i, s in enumerate(SWEEPS) if s.startswith("fluidx")]
+    ad = [sweep_ms[i] for i, s in enumerate(SWEEPS) if s.startswith("advect")]
+    fl_ms = float(np.mean(fl))
+    ad_ms = float(np.mean(ad))
+    cells = n ** 3
+    achieved = FLUIDX_BYTES_PER_CELL * cells / (fl_ms * 1e-3) / 1e9
+    m = sim.monitors()
+    t, t64 = sim.time()
+    finite = bool(np.isfinite(m).all())
+    sim.close()
+
+    out = {
+        "metric": "zone-updates/s @256^3 FP32 (plain)",
+        "value": zus,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{n}^3 periodic FP32 box, Alfven background + random-phase velocity (rms Mach 0.1) "
+                               f"+ curl-A field (rms 0.1 B0), plain method, {args.steps} timed steps",
+                   "grid": [n, n, n], "method": "plain", "precision": "fp32"},
+        "roofline": {"kernel": "fluidx", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic("fluidx", n),
+                     "bytes_per_launch": FLUIDX_BYTES_PER_CELL * cells, "avg_launch_ms": fl_ms},
+        "step_roofline": {"bytes_per_zone_update": STEP_BYTES_PER_CELL,
+                          "achieved_GBs": zus * STEP_BYTES_PER_CELL / 1e9,
+                          "frac": zus * STEP_BYTES_PER_CELL / 1e9 / HBM_PEAK_GBS},
+        "kernel_ms": {"cfl_dt": float(prof[0]), "fluidx_avg": fl_ms, "advect_avg": ad_ms,
+                      "sweeps": {f"{i:02d}_{s}": float(v) for i, (s, v) in enumerate(zip(SWEEPS, sweep_ms))}},
+        "monitors_fp64": {"mass": m[0], "energy": m[1], "max_divB": m[5], "t": t, "t64": t64, "finite": finite},
+    }
+    if not args.no_cpu:
+        log("bench: CPU baseline")
+        out["cpu_baseline"] = cpu_baseline(u0, b0, n)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -77,6 +77,30 @@ static int step_scratch(unsigned long long **p)
     return MHD_OK;
 }
 
+// per-device copy buffer of the step-level advectbyzx (the input b it reads
+// while it writes the caller's b), grown on demand
+static std::mutex g_bcopy_mu;
+static void *g_bcopy[64];
+static size_t g_bcopy_bytes[64];
+
+static int bcopy_scratch(size_t bytes, void **p)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= 64) return fail(MHD_ERR_ARG, "device %d out of range", dev);
+    if (g_bcopy_bytes[dev] < bytes) {
+        if (g_bcopy[dev]) (void)hipFree(g_bcopy[dev]);
+        g_bcopy[dev] = nullptr;
+        g_bcopy_bytes[dev] = 0;
+        e = hipMalloc(&g_bcopy[dev], bytes);
+        if (e != hipSuccess) return fail(MHD_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+        g_bcopy_bytes[dev] = bytes;
+    }
+    *p = g_bcopy[dev];
+    return MHD_OK;
+}
+
 static int sync_check(hipError_t e, const char *where)
 {
     if (e != hipSuccess) return hip_fail(e, where);
@@ -155,7 +179,14 @@ int mhd_advectbyzx(const float *u, float *b, const int *nx, const int *ny, const
     if (!u || !b || !nx || !ny || !nz || !dt) return fail(MHD_ERR_ARG, "mhd_advectbyzx: null argument");
     int rc = check_shape(*nx, *ny, *nz);
     if (rc) return rc;
-    return sync_check(launch_advect<float>(u, b, 0, *nx, *ny, *nz, dt, 0), "mhd_advectbyzx");
+    const size_t bytes = (size_t)*nx * *ny * *nz * 3 * sizeof(float);
+    std::lock_guard<std::mutex> lk(g_bcopy_mu);
+    void *bin = nullptr;
+    rc = bcopy_scratch(bytes, &bin);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(bin, b, bytes, hipMemcpyDeviceToDevice, 0);
+    if (e == hipSuccess) e = launch_advect<float>(u, (const float *)bin, b, 0, *nx, *ny, *nz, dt, 0);
+    return sync_check(e, "mhd_advectbyzx");
 }
 
 static int transpose_common(int which, float *ut, float *bt, float *u, float *b, const int *nx, const int *ny,

@@ -23,6 +23,7 @@ struct mhd_ctx {
     size_t ncell;
     hipStream_t st;
     void *u, *b;               // state (AoS, ctx precision)
+    void *b2;                  // second field buffer: advectbyzx reads one, writes the other
     unsigned long long *cmax;  // CFL max accumulator
     void *dt, *t;              // device scalars in ctx precision
     double *t64;
@@ -52,12 +53,23 @@ template <typename T> static hipError_t enqueue_cfl_dt(mhd_ctx *c)
 static const int kSweepFrame[12] = {0, 0, 1, 1, 2, 2, 2, 2, 1, 1, 0, 0};
 static const int kSweepOp[12] = {0, 1, 0, 1, 0, 1, 1, 0, 1, 0, 1, 0};  // 0 fluidx, 1 advectbyzx
 
+// The field lives in b before sweep 0; each advectbyzx moves it to the other
+// buffer.  The six advectbyzx of a step bring it back to b.
+static int field_buffer_before(int s)
+{
+    int nadv = 0;
+    for (int i = 0; i < s; i++) nadv += kSweepOp[i];
+    return nadv & 1;
+}
+
 template <typename T> static hipError_t enqueue_sweep(mhd_ctx *c, int s)
 {
     const int fr = kSweepFrame[s];
+    T *bcur = (T *)(field_buffer_before(s) ? c->b2 : c->b);
+    T *bnext = (T *)(field_buffer_before(s) ? c->b : c->b2);
     if (kSweepOp[s] == 0)
-        return launch_fluidx<T>((T *)c->u, (const T *)c->b, fr, c->nx, c->ny, c->nz, (const T *)c->dt, c->st);
-    return launch_advect<T>((const T *)c->u, (T *)c->b, fr, c->nx, c->ny, c->nz, (const T *)c->dt, c->st);
+        return launch_fluidx<T>((T *)c->u, bcur, fr, c->nx, c->ny, c->nz, (const T *)c->dt, c->st);
+    return launch_advect<T>((const T *)c->u, bcur, bnext, fr, c->nx, c->ny, c->nz, (const T *)c->dt, c->st);
 }
 
 template <typename T> static hipError_t enqueue_step(mhd_ctx *c)
@@ -135,7 +147,8 @@ int mhd_ctx_create(mhd_ctx **out, int nx, int ny, int nz, int precision, int avg
     struct A {
         void **p;
         size_t n;
-    } al[] = {{&c->u, 5 * c->ncell * P},   {&c->b, 3 * c->ncell * P}, {(void **)&c->cmax, 64},
+    } al[] = {{&c->u, 5 * c->ncell * P},   {&c->b, 3 * c->ncell * P}, {&c->b2, 3 * c->ncell * P},
+              {(void **)&c->cmax, 64},
               {&c->dt, 64},                 {&c->t, 64},               {(void **)&c->t64, 64},
               {&c->dts, (size_t)kDtCap * P}, {(void **)&c->iter, 64},  {(void **)&c->mon_part, 6 * 8 * 1024},
               {(void **)&c->mon_out, 64}};
@@ -167,7 +180,7 @@ int mhd_ctx_destroy(mhd_ctx *c)
     if (!c) return MHD_OK;
     if (c->st) (void)hipStreamSynchronize(c->st);
     drop_graph(c);
-    void *ps[] = {c->u, c->b, c->cmax, c->dt, c->t, c->t64, c->dts, c->iter, c->mon_part, c->mon_out};
+    void *ps[] = {c->u, c->b, c->b2, c->cmax, c->dt, c->t, c->t64, c->dts, c->iter, c->mon_part, c->mon_out};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -13,8 +13,10 @@ Frame make_frame(int fr, int nx, int ny, int nz);
 // sweeps over an x-major AoS box nx*ny*nz in frame fr (0: x, 1: y, 2: z)
 template <typename T>
 hipError_t launch_fluidx(T *u, const T *b, int fr, int nx, int ny, int nz, const T *dt, hipStream_t st);
+// reads bin, writes bout (must be distinct buffers)
 template <typename T>
-hipError_t launch_advect(const T *u, T *b, int fr, int nx, int ny, int nz, const T *dt, hipStream_t st);
+hipError_t launch_advect(const T *u, const T *bin, T *bout, int fr, int nx, int ny, int nz, const T *dt,
+                         hipStream_t st);
 
 // per-cell CFL speed max into *cmax (IEEE bits, atomically max-ed)
 template <typename T>

@@ -336,9 +336,14 @@ __device__ __forceinline__ void tvdb_pencil(const T (&bin)[K], const T (&vg)[K],
     }
 }
 
+// b is read from bin and written to bout (distinct buffers): a tile reads the
+// old b2 of the next pencil along j and the old b3 of the next plane, which
+// other tiles update, exactly as the reference reads its update_b copy
+// (advectbyzxA1.c:43, B1.c:43) while writing b.
 template <typename T, int K, int W, int FR>
-__global__ void __launch_bounds__(64 * W) advect_kernel(const T *__restrict__ u, T *__restrict__ b, Frame f,
-                                                        const T *__restrict__ dtp, int ntj)
+__global__ void __launch_bounds__(64 * W) advect_kernel(const T *__restrict__ u, const T *__restrict__ bin,
+                                                        T *__restrict__ bout, Frame f, const T *__restrict__ dtp,
+                                                        int ntj)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *smem = reinterpret_cast<T *>(smem_raw);
@@ -363,8 +368,8 @@ __global__ void __launch_bounds__(64 * W) advect_kernel(const T *__restrict__ u,
     load_comp<T, K, 5, W, XM>(sm + (W + 2) * PK, PK, u, pm, f, j0, nvalid, km);
     load_comp<T, K, 5, W, XM>(sr + (2 * W + 2) * PK, PK, u, 0, f, j0, nvalid, kp);
     load_comp<T, K, 5, W, XM>(sm + (2 * W + 2) * PK, PK, u, pm, f, j0, nvalid, kp);
-    load_full<T, K, 3, W + 1, XM, InvB<FR>>(sb, PB, b, f, j0, nvalid + 1, k);
-    load_comp<T, K, 3, W, XM>(sb3p, PK, b, Comp<FR>::b[2], f, j0, nvalid, kp);
+    load_full<T, K, 3, W + 1, XM, InvB<FR>>(sb, PB, bin, f, j0, nvalid + 1, k);
+    load_comp<T, K, 3, W, XM>(sb3p, PK, bin, Comp<FR>::b[2], f, j0, nvalid, kp);
     __syncthreads();
 
     const int w = threadIdx.x >> 6;
@@ -447,7 +452,7 @@ __global__ void __launch_bounds__(64 * W) advect_kernel(const T *__restrict__ u,
         }
     }
     __syncthreads();
-    store_full<T, K, 3, W, XM, InvB<FR>>(sb, PB, b, f, j0, nvalid, k);
+    store_full<T, K, 3, W, XM, InvB<FR>>(sb, PB, bout, f, j0, nvalid, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -716,7 +721,7 @@ static hipError_t fluidx_launch_k(T *u, const T *b, const Frame &f, const T *dt,
 }
 
 template <typename T, int K, int FR>
-static hipError_t advect_launch_k(const T *u, T *b, const Frame &f, const T *dt, hipStream_t st)
+static hipError_t advect_launch_k(const T *u, const T *bin, T *bout, const Frame &f, const T *dt, hipStream_t st)
 {
     constexpr int W = tile_w<T>();
     const size_t lds = sizeof(T) * advect_lds_words<T, K, W>(f.ni);
@@ -725,7 +730,7 @@ static hipError_t advect_launch_k(const T *u, T *b, const Frame &f, const T *dt,
     if (ea != hipSuccess) return ea;
     const int ntj = (f.nj + W - 1) / W;
     const int nblk = ntj * f.nk;
-    hipLaunchKernelGGL(kern, dim3(nblk), dim3(64 * W), lds, st, u, b, f, dt, ntj);
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(64 * W), lds, st, u, bin, bout, f, dt, ntj);
     return hipGetLastError();
 }
 
@@ -747,13 +752,14 @@ hipError_t launch_fluidx(T *u, const T *b, int fr, int nx, int ny, int nz, const
 }
 
 template <typename T>
-hipError_t launch_advect(const T *u, T *b, int fr, int nx, int ny, int nz, const T *dt, hipStream_t st)
+hipError_t launch_advect(const T *u, const T *bin, T *bout, int fr, int nx, int ny, int nz, const T *dt,
+                         hipStream_t st)
 {
     const Frame f = make_frame(fr, nx, ny, nz);
     const int K = pick_k(f.ni);
-#define AX0(KK) advect_launch_k<T, KK, 0>(u, b, f, dt, st)
-#define AX1(KK) advect_launch_k<T, KK, 1>(u, b, f, dt, st)
-#define AX2(KK) advect_launch_k<T, KK, 2>(u, b, f, dt, st)
+#define AX0(KK) advect_launch_k<T, KK, 0>(u, bin, bout, f, dt, st)
+#define AX1(KK) advect_launch_k<T, KK, 1>(u, bin, bout, f, dt, st)
+#define AX2(KK) advect_launch_k<T, KK, 2>(u, bin, bout, f, dt, st)
     if (fr == 0) { MHD_KSWITCH(K, AX0) }
     if (fr == 1) { MHD_KSWITCH(K, AX1) }
     MHD_KSWITCH(K, AX2)
@@ -838,7 +844,7 @@ hipError_t launch_monitors(const T *u, const T *b, int nx, int ny, int nz, doubl
 
 #define MHD_INST(T)                                                                                                  \
     template hipError_t launch_fluidx<T>(T *, const T *, int, int, int, int, const T *, hipStream_t);              \
-    template hipError_t launch_advect<T>(const T *, T *, int, int, int, int, const T *, hipStream_t);              \
+    template hipError_t launch_advect<T>(const T *, const T *, T *, int, int, int, int, const T *, hipStream_t);              \
     template hipError_t launch_cfl<T>(const T *, const T *, int, int, int, int, unsigned long long *, hipStream_t); \
     template hipError_t launch_dt<T>(unsigned long long *, T *, T *, double *, T, T *, int *, int, hipStream_t);    \
     template hipError_t launch_cfl_value<T>(unsigned long long *, T *, hipStream_t);                               \

@@ -133,7 +133,8 @@ def test_config1_alfven_32cube_200_steps(M, oracle):
 
 @pytest.mark.parametrize("shape,seed,amp,steps,tf", [((24, 16, 32), 501, 0.4, 5, math.inf),
                                                      ((16, 16, 16), 502, 0.55, 6, 3.0),
-                                                     ((48, 96, 64), 503, 0.3, 2, math.inf)])
+                                                     ((48, 96, 64), 503, 0.3, 2, math.inf),
+                                                     ((42, 22, 26), 504, 0.45, 3, math.inf)])
 def test_ctx_steps_match_oracle(M, oracle, shape, seed, amp, steps, tf):
     nx, ny, nz = shape
     u0, b0 = random_state(nx, ny, nz, seed=seed, amp=amp)
@@ -225,9 +226,11 @@ def test_full_size_256_step_properties_and_parity(M, oracle):
     sim.step(1)
     ug, bg = sim.get_state()
     m1 = sim.monitors()
+    dg = sim.dts()
     sim.close()
-    assert m0[5] < 1e-5 and m1[5] < 1e-5  # div B from curl A stays at rounding level
-    assert abs(m1[0] - m0[0]) <= 1e-6 * m0[0]
     uo, bo = u0.copy(), b0.copy()
     do = oracle.run(uo, bo, n, n, n, 1)
+    assert bitwise(dg, do)
     assert same(ug, uo) and same(bg, bo)
+    assert m0[5] < 1e-5 and m1[5] < 1e-5  # div B from curl A stays at rounding level
+    assert abs(m1[0] - m0[0]) <= 1e-6 * m0[0]
