@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", choices=["plain", "copy"], default="plain",
+                    help="copy: time the plain copy kernel on 1 GiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it", file=sys.stderr)
@@ -84,6 +86,19 @@ def main():
 
     import mhdgpu
     from mhdgpu.ics import random_phase
+
+    if args.mode == "copy":
+        nbytes = 1 << 30
+        src, dst = mhdgpu.DeviceArray(nbytes, np.uint8), mhdgpu.DeviceArray(nbytes, np.uint8)
+        for _ in range(args.warmup):
+            mhdgpu.check(mhdgpu.lib().mhd_dev_copy_kernel(dst.ptr, src.ptr, nbytes))
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            mhdgpu.check(mhdgpu.lib().mhd_dev_copy_kernel(dst.ptr, src.ptr, nbytes))
+        wall = time.perf_counter() - t0
+        print(json.dumps({"mode": "copy", "bytes_copied": nbytes, "launches": args.steps,
+                          "GBs_read_plus_write": 2 * nbytes * args.steps / wall / 1e9}), flush=True)
+        return
 
     n = args.n
     log(f"bench: generating {n}^3 initial state")

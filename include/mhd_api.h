@@ -43,6 +43,9 @@ int mhd_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int mhd_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int mhd_memcpy_d2d(void *dst, const void *src, size_t bytes);
 int mhd_device_sync(void);
+/* device-to-device copy by a plain streaming kernel (bytes % 16 == 0), the
+ * yardstick for HBM byte counters; no counterpart in the reference */
+int mhd_dev_copy_kernel(void *dst, const void *src, size_t bytes);
 /* free and total device memory in bytes */
 int mhd_mem_info(size_t *free_bytes, size_t *total_bytes);
 
@@ -139,7 +142,9 @@ int mhd_ctx_dts(mhd_ctx *ctx, double *h_dts, int max, int *count);
 int mhd_ctx_time(mhd_ctx *ctx, double *t, double *t64);
 /* FP64 monitors: out[6] = total mass, total energy, momentum x, y, z, max |div B| */
 int mhd_ctx_monitors(mhd_ctx *ctx, double *out6);
-/* device pointers of the state, for zero-copy callers */
+/* device pointers of the state, for zero-copy callers: the context keeps the
+ * state as structure of arrays, d_u = 5 planes (rho, rho vx, rho vy, rho vz,
+ * E) and d_b = 3 planes (bx, by, bz) of nx*ny*nz values each, x fastest */
 int mhd_ctx_device_state(mhd_ctx *ctx, void **d_u, void **d_b);
 /* time per kernel of one step, for the roofline: runs `reps` steps with
  * events around every launch; ms_out[0..12] = cfl+dt, then the 12 sweeps in

@@ -69,8 +69,8 @@ static int step_scratch(unsigned long long **p)
     if (!g_scr[dev]) {
         e = hipMalloc((void **)&g_scr[dev], 64);
         if (e != hipSuccess) return fail(MHD_ERR_NOMEM, "hipMalloc scratch: %s", hipGetErrorString(e));
-        launch_zero_u64(g_scr[dev], 8, 0);
-        e = hipDeviceSynchronize();
+        e = launch_zero_u64(g_scr[dev], 8, 0);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
         if (e != hipSuccess) return hip_fail(e, "scratch init");
     }
     *p = g_scr[dev];
@@ -148,6 +148,12 @@ int mhd_memcpy_d2d(void *dst, const void *src, size_t bytes)
 {
     hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice);
     return e == hipSuccess ? MHD_OK : hip_fail(e, "hipMemcpy D2D");
+}
+
+int mhd_dev_copy_kernel(void *dst, const void *src, size_t bytes)
+{
+    if (!dst || !src || bytes % 16) return fail(MHD_ERR_ARG, "mhd_dev_copy_kernel: bad argument");
+    return sync_check(launch_copy(dst, src, bytes, 0), "mhd_dev_copy_kernel");
 }
 
 int mhd_device_sync(void)

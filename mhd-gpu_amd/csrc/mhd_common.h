@@ -58,6 +58,17 @@ template <> struct CompInv<2> {
     static constexpr int b[3] = {1, 2, 0};
 };
 
+// physical plane of frame component c, as selects (c may vary per lane)
+template <int FR> __device__ __forceinline__ int cu(int c)
+{
+    return c == 0 ? Comp<FR>::u[0]
+                  : c == 1 ? Comp<FR>::u[1] : c == 2 ? Comp<FR>::u[2] : c == 3 ? Comp<FR>::u[3] : Comp<FR>::u[4];
+}
+template <int FR> __device__ __forceinline__ int cb(int c)
+{
+    return c == 0 ? Comp<FR>::b[0] : c == 1 ? Comp<FR>::b[1] : Comp<FR>::b[2];
+}
+
 template <typename T> __device__ __forceinline__ T gamma_c() { return (T)(5.0 / 3.0); }
 
 // van Leer harmonic mean: the test is a product in T, the value

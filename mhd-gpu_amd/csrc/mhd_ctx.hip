@@ -15,6 +15,7 @@
 
 #include "../../include/mhd_api.h"
 #include "mhd_internal.h"
+#include "mhd_soa.h"
 
 using namespace mhd;
 
@@ -22,7 +23,7 @@ struct mhd_ctx {
     int nx, ny, nz, prec, avg;
     size_t ncell;
     hipStream_t st;
-    void *u, *b;               // state (AoS, ctx precision)
+    void *u, *b;               // state: 5 + 3 planes of ncell values (SoA, ctx precision)
     void *b2;                  // second field buffer: advectbyzx reads one, writes the other
     unsigned long long *cmax;  // CFL max accumulator
     void *dt, *t;              // device scalars in ctx precision
@@ -43,7 +44,7 @@ static const int kDtCap = 1 << 20;
 
 template <typename T> static hipError_t enqueue_cfl_dt(mhd_ctx *c)
 {
-    hipError_t e = launch_cfl<T>((const T *)c->u, (const T *)c->b, c->nx, c->ny, c->nz, c->avg, c->cmax, c->st);
+    hipError_t e = launch_cfl_soa<T>((const T *)c->u, (const T *)c->b, c->nx, c->ny, c->nz, c->avg, c->cmax, c->st);
     if (e == hipSuccess)
         e = launch_dt<T>(c->cmax, (T *)c->dt, (T *)c->t, c->t64, (T)c->tf, (T *)c->dts, c->iter, c->cap, c->st);
     return e;
@@ -68,8 +69,8 @@ template <typename T> static hipError_t enqueue_sweep(mhd_ctx *c, int s)
     T *bcur = (T *)(field_buffer_before(s) ? c->b2 : c->b);
     T *bnext = (T *)(field_buffer_before(s) ? c->b : c->b2);
     if (kSweepOp[s] == 0)
-        return launch_fluidx<T>((T *)c->u, bcur, fr, c->nx, c->ny, c->nz, (const T *)c->dt, c->st);
-    return launch_advect<T>((const T *)c->u, bcur, bnext, fr, c->nx, c->ny, c->nz, (const T *)c->dt, c->st);
+        return launch_fluidx_soa<T>((T *)c->u, bcur, fr, c->nx, c->ny, c->nz, (const T *)c->dt, c->st);
+    return launch_advect_soa<T>((const T *)c->u, bcur, bnext, fr, c->nx, c->ny, c->nz, (const T *)c->dt, c->st);
 }
 
 template <typename T> static hipError_t enqueue_step(mhd_ctx *c)
@@ -161,7 +162,7 @@ int mhd_ctx_create(mhd_ctx **out, int nx, int ny, int nz, int precision, int avg
     }
     e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
-    if (e == hipSuccess) e = c->prec == 4 ? prepare_sweeps<float>(nx, ny, nz) : prepare_sweeps<double>(nx, ny, nz);
+    if (e == hipSuccess) e = c->prec == 4 ? prepare_soa<float>(nx, ny, nz) : prepare_soa<double>(nx, ny, nz);
     if (e != hipSuccess) {
         mhd_ctx_destroy(c);
         return hip_fail(e, "mhd_ctx_create");
@@ -190,22 +191,49 @@ int mhd_ctx_destroy(mhd_ctx *c)
     return MHD_OK;
 }
 
+// host AoS <-> device SoA through a temporary device buffer
+static int transfer_state(mhd_ctx *c, void *h_u, void *h_b, int to_device)
+{
+    const size_t bu = 5 * c->ncell * c->prec, bb = 3 * c->ncell * c->prec;
+    void *tmp = nullptr;
+    hipError_t e = hipMalloc(&tmp, bu);
+    if (e != hipSuccess) return fail(MHD_ERR_NOMEM, "state transfer buffer: %s", hipGetErrorString(e));
+    const long N = (long)c->ncell;
+    struct Part {
+        void *host, *dev;
+        size_t bytes;
+        int ncomp;
+    } parts[2] = {{h_u, c->u, bu, 5}, {h_b, c->b, bb, 3}};
+    for (auto &p : parts) {
+        if (e != hipSuccess) break;
+        if (to_device) {
+            e = hipMemcpyAsync(tmp, p.host, p.bytes, hipMemcpyHostToDevice, c->st);
+            if (e == hipSuccess)
+                e = c->prec == 4 ? launch_aos_soa<float>((float *)p.dev, (const float *)tmp, p.ncomp, N, 1, c->st)
+                                 : launch_aos_soa<double>((double *)p.dev, (const double *)tmp, p.ncomp, N, 1, c->st);
+        } else {
+            e = c->prec == 4 ? launch_aos_soa<float>((float *)tmp, (const float *)p.dev, p.ncomp, N, 0, c->st)
+                             : launch_aos_soa<double>((double *)tmp, (const double *)p.dev, p.ncomp, N, 0, c->st);
+            if (e == hipSuccess) e = hipMemcpyAsync(p.host, tmp, p.bytes, hipMemcpyDeviceToHost, c->st);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    }
+    (void)hipFree(tmp);
+    return e == hipSuccess ? MHD_OK : hip_fail(e, to_device ? "mhd_ctx_set_state" : "mhd_ctx_get_state");
+}
+
 int mhd_ctx_set_state(mhd_ctx *c, const void *h_u, const void *h_b)
 {
     if (!c || !h_u || !h_b) return fail(MHD_ERR_ARG, "mhd_ctx_set_state: null argument");
-    hipError_t e = hipMemcpyAsync(c->u, h_u, 5 * c->ncell * c->prec, hipMemcpyHostToDevice, c->st);
-    if (e == hipSuccess) e = hipMemcpyAsync(c->b, h_b, 3 * c->ncell * c->prec, hipMemcpyHostToDevice, c->st);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
-    return e == hipSuccess ? MHD_OK : hip_fail(e, "mhd_ctx_set_state");
+    hipError_t e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) return hip_fail(e, "mhd_ctx_set_state");
+    return transfer_state(c, const_cast<void *>(h_u), const_cast<void *>(h_b), 1);
 }
 
 int mhd_ctx_get_state(mhd_ctx *c, void *h_u, void *h_b)
 {
     if (!c || !h_u || !h_b) return fail(MHD_ERR_ARG, "mhd_ctx_get_state: null argument");
-    hipError_t e = hipMemcpyAsync(h_u, c->u, 5 * c->ncell * c->prec, hipMemcpyDeviceToHost, c->st);
-    if (e == hipSuccess) e = hipMemcpyAsync(h_b, c->b, 3 * c->ncell * c->prec, hipMemcpyDeviceToHost, c->st);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
-    return e == hipSuccess ? MHD_OK : hip_fail(e, "mhd_ctx_get_state");
+    return transfer_state(c, h_u, h_b, 0);
 }
 
 int mhd_ctx_reset_clock(mhd_ctx *c, double tf)
@@ -302,10 +330,11 @@ int mhd_ctx_time(mhd_ctx *c, double *t, double *t64)
 int mhd_ctx_monitors(mhd_ctx *c, double *out6)
 {
     if (!c || !out6) return fail(MHD_ERR_ARG, "mhd_ctx_monitors: null argument");
-    hipError_t e = c->prec == 4 ? launch_monitors<float>((const float *)c->u, (const float *)c->b, c->nx, c->ny, c->nz,
-                                                          c->mon_part, c->mon_nblk, c->mon_out, c->st)
-                                : launch_monitors<double>((const double *)c->u, (const double *)c->b, c->nx, c->ny,
-                                                          c->nz, c->mon_part, c->mon_nblk, c->mon_out, c->st);
+    hipError_t e = c->prec == 4
+                       ? launch_monitors_soa<float>((const float *)c->u, (const float *)c->b, c->nx, c->ny, c->nz,
+                                                    c->mon_part, c->mon_nblk, c->mon_out, c->st)
+                       : launch_monitors_soa<double>((const double *)c->u, (const double *)c->b, c->nx, c->ny, c->nz,
+                                                     c->mon_part, c->mon_nblk, c->mon_out, c->st);
     if (e == hipSuccess) e = hipMemcpyAsync(out6, c->mon_out, 6 * sizeof(double), hipMemcpyDeviceToHost, c->st);
     if (e == hipSuccess) e = hipStreamSynchronize(c->st);
     return e == hipSuccess ? MHD_OK : hip_fail(e, "mhd_ctx_monitors");

@@ -29,11 +29,15 @@ hipError_t launch_dt(unsigned long long *cmax, T *dt, T *t, double *t64, T tf, T
 template <typename T> hipError_t launch_cfl_value(unsigned long long *cmax, T *out, hipStream_t st);
 template <typename T> hipError_t prepare_sweeps(int nx, int ny, int nz);
 hipError_t launch_zero_u64(unsigned long long *p, int n, hipStream_t st);
+// plain device copy kernel (bytes a multiple of 16)
+hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st);
 
 // which = 12 or 13: out-of-place transposes with the reference's component maps
 template <typename T>
 hipError_t launch_transpose(T *uo, T *bo, const T *u, const T *b, int which, int nx, int ny, int nz, hipStream_t st);
 
+// deterministic final reduction of per-block monitor partials (6 per block)
+hipError_t launch_monitor_final(const double *part, int nblk, double *out, hipStream_t st);
 // FP64 monitors: out[6] = mass, energy, momentum x, y, z, max |div B|
 template <typename T>
 hipError_t launch_monitors(const T *u, const T *b, int nx, int ny, int nz, double *part, int nblk, double *out,

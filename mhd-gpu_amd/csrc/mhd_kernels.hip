@@ -11,6 +11,7 @@
 // Neighbours along the pencil come from lane shuffles; the pencil-global max
 // of the freezing speed is a wave reduction.
 #include "mhd_kernels.h"
+#include "mhd_core.h"
 
 namespace mhd {
 
@@ -135,88 +136,6 @@ template <typename T, int K, int W> __host__ __device__ constexpr int fluidx_lds
     return W * plen<K, 5>(n) + (W + 1) * plen<K, 3>(n) + W * plen<K, 1>(n);
 }
 
-template <typename T, int K>
-__device__ __forceinline__ void mhdflux_pencil(const T (&uu)[K][5], const T (&bx)[K][3], T (&v)[K][5], T &c)
-{
-    T cl = (T)0;
-#pragma unroll
-    for (int q = 0; q < K; q++) {
-        T ci;
-        mhdflux_cell<T>(uu[q], bx[q], v[q], ci);
-        cl = fmax(cl, ci);
-    }
-    c = wave_max(cl);
-    if (c > (T)0) {
-#pragma unroll
-        for (int q = 0; q < K; q++)
-#pragma unroll
-            for (int m = 0; m < 5; m++) v[q][m] = v[q][m] / c;
-    }
-}
-
-// tvd1 on a pencil held K cells per lane; uu is updated in place.
-template <typename T, int K>
-__device__ __forceinline__ void tvd1_pencil(T (&uu)[K][5], const T (&bx)[K][3], T dt, int src_next, int src_prev)
-{
-    const T h = (T)(-1.0 * (double)dt / 2.0);
-    const T mdt = (T)(-1.0 * (double)dt);
-    T v[K][5], c;
-    T u1[K][5];
-    // first half step (tvd1.c:34-49)
-    mhdflux_pencil<T, K>(uu, bx, v, c);
-#pragma unroll
-    for (int m = 0; m < 5; m++) {
-        T wl[K], fr[K], wln[K], fl[K], fx[K], fxp[K];
-#pragma unroll
-        for (int q = 0; q < K; q++) {
-            const T wr = uu[q][m] + v[q][m];
-            wl[q] = uu[q][m] - v[q][m];
-            fr[q] = c * wr;
-        }
-        shift_next<T, K>(wl, wln, src_next);
-#pragma unroll
-        for (int q = 0; q < K; q++) {
-            fl[q] = c * wln[q];
-            fx[q] = (T)0.5 * fr[q] + (T)(-0.5) * fl[q];
-        }
-        shift_prev<T, K>(fx, fxp, src_prev);
-#pragma unroll
-        for (int q = 0; q < K; q++) u1[q][m] = uu[q][m] + h * (fx[q] - fxp[q]);
-    }
-    // full step with limited fluxes (tvd1.c:52-116)
-    mhdflux_pencil<T, K>(u1, bx, v, c);
-#pragma unroll
-    for (int m = 0; m < 5; m++) {
-        T wl[K], fr[K], frn[K], frp[K], wln[K], fl[K], fln[K], flp[K], fx[K], fxp[K];
-#pragma unroll
-        for (int q = 0; q < K; q++) {
-            const T wr = u1[q][m] + v[q][m];
-            wl[q] = u1[q][m] - v[q][m];
-            fr[q] = c * wr;
-        }
-        shift_next<T, K>(fr, frn, src_next);
-        shift_prev<T, K>(fr, frp, src_prev);
-        shift_next<T, K>(wl, wln, src_next);
-#pragma unroll
-        for (int q = 0; q < K; q++) fl[q] = c * wln[q];
-        shift_next<T, K>(fl, fln, src_next);
-        shift_prev<T, K>(fl, flp, src_prev);
-#pragma unroll
-        for (int q = 0; q < K; q++) {
-            const T dfrp = (T)0.5 * frn[q] + (T)(-0.5) * fr[q];
-            const T dfrm = (T)0.5 * fr[q] + (T)(-0.5) * frp[q];
-            const T dflp = (T)0.5 * fl[q] + (T)(-0.5) * fln[q];
-            const T dflm = (T)0.5 * flp[q] + (T)(-0.5) * fl[q];
-            const T dfr = vanleer<T>(dfrp, dfrm);
-            const T dfl = vanleer<T>(dflp, dflm);
-            fx[q] = (T)0.5 * (fr[q] - fl[q]) + (T)0.5 * (dfr - dfl);
-        }
-        shift_prev<T, K>(fx, fxp, src_prev);
-#pragma unroll
-        for (int q = 0; q < K; q++) uu[q][m] = uu[q][m] + mdt * (fx[q] - fxp[q]);
-    }
-}
-
 template <typename T, int K, int W, int FR>
 __global__ void __launch_bounds__(64 * W) fluidx_kernel(T *__restrict__ u, const T *__restrict__ b, Frame f,
                                                         const T *__restrict__ dtp, int ntj)
@@ -286,54 +205,6 @@ template <typename T, int K, int W> __host__ __device__ constexpr int advect_lds
     // b3 at k+1: W slots; fluxes FA (W+1), FB0 (W), FB1 (W)
     return 2 * (3 * W + 2) * plen<K, 1>(n) + (W + 1) * plen<K, 3>(n) + W * plen<K, 1>(n) +
            (3 * W + 1) * plen<K, 1>(n);
-}
-
-// tvdb on a pencil held K cells per lane (tvdb.c:10-91): bin -> bnew, and
-// F[i] = flux[i-1], the shifted flux the CT update uses.
-template <typename T, int K>
-__device__ __forceinline__ void tvdb_pencil(const T (&bin)[K], const T (&vg)[K], T dt, T (&bnew)[K], T (&F)[K],
-                                            int src_next, int src_prev)
-{
-    T vgn[K], vh[K], bv[K], bvn[K], f1[K], f1p[K], b1[K], g[K], gn[K], gnn[K], gp[K], fl[K], flp[K];
-    shift_next<T, K>(vg, vgn, src_next);
-#pragma unroll
-    for (int q = 0; q < K; q++) {
-        vh[q] = (T)0.5 * vg[q] + (T)0.5 * vgn[q];
-        bv[q] = bin[q] * vg[q];
-    }
-    shift_next<T, K>(bv, bvn, src_next);
-#pragma unroll
-    for (int q = 0; q < K; q++) f1[q] = vh[q] > (T)0 ? bv[q] : bvn[q];
-    shift_prev<T, K>(f1, f1p, src_prev);
-#pragma unroll
-    for (int q = 0; q < K; q++) {
-        b1[q] = (T)((double)bin[q] - (double)((f1[q] - f1p[q]) * dt) / 2.0);
-        g[q] = vg[q] * b1[q];
-    }
-    shift_next<T, K>(g, gn, src_next);
-    shift_next<T, K>(gn, gnn, src_next);
-    shift_prev<T, K>(g, gp, src_prev);
-#pragma unroll
-    for (int q = 0; q < K; q++) {
-        T w, wp, wm;
-        if (vh[q] > (T)0) {
-            w = g[q];
-            wp = (T)((double)(gn[q] - w) / 2.0);
-            wm = (T)((double)(w - gp[q]) / 2.0);
-        } else {
-            w = gn[q];
-            wp = (T)((double)(w - gnn[q]) / 2.0);
-            wm = (T)((double)(g[q] - w) / 2.0);
-        }
-        const T dw = vanleer<T>(wm, wp);
-        fl[q] = (w + dw) * dt;
-    }
-    shift_prev<T, K>(fl, flp, src_prev);
-#pragma unroll
-    for (int q = 0; q < K; q++) {
-        bnew[q] = bin[q] - (fl[q] - flp[q]);
-        F[q] = flp[q];
-    }
 }
 
 // b is read from bin and written to bout (distinct buffers): a tile reads the
@@ -539,6 +410,21 @@ template <typename T> __global__ void cfl_value_kernel(unsigned long long *cmax,
     const T c = frombits<T>(*cmax);
     *cmax = 0ull;
     *out = (T)(1.0 / (double)c);
+}
+
+// plain streaming copy (16 B per lane per iteration), the calibration
+// yardstick of the HBM byte counters
+__global__ void __launch_bounds__(256) copy_kernel(uint4 *__restrict__ dst, const uint4 *__restrict__ src, size_t n16)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st)
+{
+    const size_t n16 = bytes / 16;
+    hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, st, (uint4 *)dst, (const uint4 *)src, n16);
+    return hipGetLastError();
 }
 
 __global__ void zero_u64_kernel(unsigned long long *p, int n)
@@ -833,13 +719,18 @@ hipError_t launch_transpose(T *uo, T *bo, const T *u, const T *b, int which, int
     return hipGetLastError();
 }
 
+hipError_t launch_monitor_final(const double *part, int nblk, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(monitor_final_kernel, dim3(1), dim3(64), 0, st, part, nblk, out);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_monitors(const T *u, const T *b, int nx, int ny, int nz, double *part, int nblk, double *out,
                            hipStream_t st)
 {
     hipLaunchKernelGGL(monitor_partial_kernel<T>, dim3(nblk), dim3(256), 0, st, u, b, nx, ny, nz, part);
-    hipLaunchKernelGGL(monitor_final_kernel, dim3(1), dim3(64), 0, st, part, nblk, out);
-    return hipGetLastError();
+    return launch_monitor_final(part, nblk, out, st);
 }
 
 #define MHD_INST(T)                                                                                                  \

@@ -1,0 +1,41 @@
+// Launchers of the structure-of-arrays sweeps (mhd_soa.hip), used by the
+// device-resident context, the batched form and the server.  State: u as 5
+// planes and b as 3 planes of N = nx*ny*nz cells, x fastest.
+#pragma once
+#include "mhd_kernels.h"
+
+namespace mhd {
+
+// floor(f / d) as (f * m) >> 32, exact for 0 <= f < 2^32 / d
+struct FastDiv {
+    unsigned long long m;
+};
+FastDiv make_fastdiv(int d);
+
+// one box in one sweep frame
+struct SoaBox {
+    Frame f;
+    long N;           // plane stride in cells
+    FastDiv n_div;    // division by the pencil length
+    FastDiv n4_div;   // division by the pencil length in 16-byte vectors
+    bool vec_ok;      // 16-byte vector access applies (nx and n multiples of 16/sizeof(T))
+};
+template <typename T> SoaBox make_soa_box(int fr, int nx, int ny, int nz);
+
+template <typename T>
+hipError_t launch_fluidx_soa(T *u, const T *b, int fr, int nx, int ny, int nz, const T *dt, hipStream_t st);
+// reads bin, writes bout (distinct buffers)
+template <typename T>
+hipError_t launch_advect_soa(const T *u, const T *bin, T *bout, int fr, int nx, int ny, int nz, const T *dt,
+                             hipStream_t st);
+template <typename T> hipError_t prepare_soa(int nx, int ny, int nz);
+template <typename T>
+hipError_t launch_cfl_soa(const T *u, const T *b, int nx, int ny, int nz, int avg, unsigned long long *cmax,
+                          hipStream_t st);
+template <typename T>
+hipError_t launch_monitors_soa(const T *u, const T *b, int nx, int ny, int nz, double *part, int nblk, double *out,
+                               hipStream_t st);
+// ncomp 5 or 3; to_soa 1: AoS src -> SoA dst, 0: SoA src -> AoS dst
+template <typename T> hipError_t launch_aos_soa(T *dst, const T *src, int ncomp, long N, int to_soa, hipStream_t st);
+
+}  // namespace mhd

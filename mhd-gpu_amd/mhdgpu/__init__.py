@@ -41,6 +41,7 @@ SIGNATURES = {
     "mhd_memcpy_d2h": (_I, [_P, _P, _SZ]),
     "mhd_memcpy_d2d": (_I, [_P, _P, _SZ]),
     "mhd_device_sync": (_I, []),
+    "mhd_dev_copy_kernel": (_I, [_P, _P, _SZ]),
     "mhd_mem_info": (_I, [_SZP, _SZP]),
     "mhd_fluidx": (_I, [_P, _P, _IP, _IP, _IP, _P]),
     "mhd_advectbyzx": (_I, [_P, _P, _IP, _IP, _IP, _P]),
