@@ -144,8 +144,11 @@ int mhd_ctx_time(mhd_ctx *ctx, double *t, double *t64);
 int mhd_ctx_monitors(mhd_ctx *ctx, double *out6);
 /* device pointers of the state, for zero-copy callers: the context keeps the
  * state as structure of arrays, d_u = 5 planes (rho, rho vx, rho vy, rho vz,
- * E) and d_b = 3 planes (bx, by, bz) of nx*ny*nz values each, x fastest */
+ * E) and d_b = 3 planes (bx, by, bz), each laid out as mhd_ctx_layout says */
 int mhd_ctx_device_state(mhd_ctx *ctx, void **d_u, void **d_b);
+/* layout of each plane: cell (i,j,k) at i + j*sy + k*sz, planes `plane` values
+ * apart (sy >= nx, sz >= sy*ny: padded rows/planes avoid power-of-two strides) */
+int mhd_ctx_layout(mhd_ctx *ctx, long *sy, long *sz, long *plane);
 /* time per kernel of one step, for the roofline: runs `reps` steps with
  * events around every launch; ms_out[0..12] = cfl+dt, then the 12 sweeps in
  * order (fluidx x, advect x, fluidx y, advect y, fluidx z, advect z,

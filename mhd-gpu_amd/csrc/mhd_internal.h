@@ -12,6 +12,9 @@ int hip_fail(hipError_t e, const char *where);
 int check_shape(int nx, int ny, int nz);
 // the same for all three sweep directions of a box
 int check_shape3(int nx, int ny, int nz);
+// default SoA padding (cells) of rows and planes for a precision (4 or 8)
+int soa_pad_y(int prec);
+int soa_pad_z(int prec);
 // set the dynamic-LDS limit of the sweep kernels a box will use, so that no
 // attribute call happens inside a stream capture
 template <typename T> hipError_t prepare_sweeps(int nx, int ny, int nz);

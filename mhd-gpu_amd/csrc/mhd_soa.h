@@ -12,6 +12,15 @@ struct FastDiv {
 };
 FastDiv make_fastdiv(int d);
 
+// SoA layout of one box: cell (i,j,k) at i + j*sy + k*sz of each plane,
+// planes N apart; sy >= nx and sz >= sy*ny leave padding that breaks
+// power-of-two strides
+struct SoaGrid {
+    int nx, ny, nz;
+    long sy, sz, N;
+};
+SoaGrid make_grid(int nx, int ny, int nz, int pad_y, int pad_z);
+
 // one box in one sweep frame
 struct SoaBox {
     Frame f;
@@ -20,22 +29,21 @@ struct SoaBox {
     FastDiv n4_div;   // division by the pencil length in 16-byte vectors
     bool vec_ok;      // 16-byte vector access applies (nx and n multiples of 16/sizeof(T))
 };
-template <typename T> SoaBox make_soa_box(int fr, int nx, int ny, int nz);
+template <typename T> SoaBox make_soa_box(int fr, const SoaGrid &g);
 
 template <typename T>
-hipError_t launch_fluidx_soa(T *u, const T *b, int fr, int nx, int ny, int nz, const T *dt, hipStream_t st);
+hipError_t launch_fluidx_soa(T *u, const T *b, int fr, const SoaGrid &g, const T *dt, hipStream_t st);
 // reads bin, writes bout (distinct buffers)
 template <typename T>
-hipError_t launch_advect_soa(const T *u, const T *bin, T *bout, int fr, int nx, int ny, int nz, const T *dt,
+hipError_t launch_advect_soa(const T *u, const T *bin, T *bout, int fr, const SoaGrid &g, const T *dt,
                              hipStream_t st);
 template <typename T> hipError_t prepare_soa(int nx, int ny, int nz);
 template <typename T>
-hipError_t launch_cfl_soa(const T *u, const T *b, int nx, int ny, int nz, int avg, unsigned long long *cmax,
-                          hipStream_t st);
+hipError_t launch_cfl_soa(const T *u, const T *b, const SoaGrid &g, int avg, unsigned long long *cmax, hipStream_t st);
 template <typename T>
-hipError_t launch_monitors_soa(const T *u, const T *b, int nx, int ny, int nz, double *part, int nblk, double *out,
+hipError_t launch_monitors_soa(const T *u, const T *b, const SoaGrid &g, double *part, int nblk, double *out,
                                hipStream_t st);
-// ncomp 5 or 3; to_soa 1: AoS src -> SoA dst, 0: SoA src -> AoS dst
-template <typename T> hipError_t launch_aos_soa(T *dst, const T *src, int ncomp, long N, int to_soa, hipStream_t st);
+// ncomp 5 or 3; to_soa 1: aos -> soa planes, 0: soa planes -> aos
+template <typename T> hipError_t launch_aos_soa(T *soa, T *aos, int ncomp, const SoaGrid &g, int to_soa, hipStream_t st);
 
 }  // namespace mhd

@@ -561,29 +561,32 @@ template <typename T> __device__ __forceinline__ unsigned long long soa_bits(T x
 }
 
 template <typename T, bool AVG>
-__global__ void __launch_bounds__(256) cfl_soa_kernel(const T *__restrict__ u, const T *__restrict__ b, int nx,
-                                                      int ny, int nz, long N, unsigned long long *cmax)
+__global__ void __launch_bounds__(256) cfl_soa_kernel(const T *__restrict__ u, const T *__restrict__ b, SoaGrid g,
+                                                      unsigned long long *cmax)
 {
+    const long N = g.N;
     T c = (T)0;
-    for (long s = (long)blockIdx.x * blockDim.x + threadIdx.x; s < N; s += (long)gridDim.x * blockDim.x) {
-        const T b1 = b[s], b2 = b[N + s], b3 = b[2 * N + s];
-        T B;
-        if constexpr (AVG) {
-            const int i = (int)(s % nx);
-            const long r = s / nx;
-            const int j = (int)(r % ny), k = (int)(r / ny);
-            const long sip = s - i + (i + 1 == nx ? 0 : i + 1);
-            const long sjp = ((long)k * ny + (j + 1 == ny ? 0 : j + 1)) * nx + i;
-            const long skp = ((long)(k + 1 == nz ? 0 : k + 1) * ny + j) * nx + i;
-            const T bxa = (T)((double)(b1 + b[sip]) / 2.0);
-            const T bya = (T)((double)(b2 + b[N + sjp]) / 2.0);
-            const T bza = (T)((double)(b3 + b[2 * N + skp]) / 2.0);
-            B = bxa * bxa + bya * bya + bza * bza;
-        } else {
-            B = b1 * b1 + b2 * b2 + b3 * b3;
+    for (int r = blockIdx.x; r < g.ny * g.nz; r += gridDim.x) {
+        const int j = r % g.ny, k = r / g.ny;
+        const long row = (long)j * g.sy + (long)k * g.sz;
+        for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
+            const long s = row + i;
+            const T b1 = b[s], b2 = b[N + s], b3 = b[2 * N + s];
+            T B;
+            if constexpr (AVG) {
+                const long sip = row + (i + 1 == g.nx ? 0 : i + 1);
+                const long sjp = (long)(j + 1 == g.ny ? 0 : j + 1) * g.sy + (long)k * g.sz + i;
+                const long skp = (long)j * g.sy + (long)(k + 1 == g.nz ? 0 : k + 1) * g.sz + i;
+                const T bxa = (T)((double)(b1 + b[sip]) / 2.0);
+                const T bya = (T)((double)(b2 + b[N + sjp]) / 2.0);
+                const T bza = (T)((double)(b3 + b[2 * N + skp]) / 2.0);
+                B = bxa * bxa + bya * bya + bza * bza;
+            } else {
+                B = b1 * b1 + b2 * b2 + b3 * b3;
+            }
+            const T sp = cfl_speed<T>(u[s], u[N + s], u[2 * N + s], u[3 * N + s], u[4 * N + s], B);
+            c = c > sp ? c : sp;  // max_2num (calcfl.c:55)
         }
-        const T sp = cfl_speed<T>(u[s], u[N + s], u[2 * N + s], u[3 * N + s], u[4 * N + s], B);
-        c = c > sp ? c : sp;
     }
     c = wave_max(c);
     __shared__ T red[4];
@@ -598,25 +601,28 @@ __global__ void __launch_bounds__(256) cfl_soa_kernel(const T *__restrict__ u, c
 
 // FP64 monitors over the SoA state (mass, energy, momentum x/y/z, max |div B|)
 template <typename T>
-__global__ void __launch_bounds__(256) monitor_soa_kernel(const T *__restrict__ u, const T *__restrict__ b, int nx,
-                                                          int ny, int nz, long N, double *part)
+__global__ void __launch_bounds__(256) monitor_soa_kernel(const T *__restrict__ u, const T *__restrict__ b, SoaGrid g,
+                                                          double *part)
 {
+    const long N = g.N;
     double a[6] = {0, 0, 0, 0, 0, 0};
-    for (long s = (long)blockIdx.x * blockDim.x + threadIdx.x; s < N; s += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(s % nx);
-        const long r = s / nx;
-        const int j = (int)(r % ny), k = (int)(r / ny);
-        const long sip = s - i + (i + 1 == nx ? 0 : i + 1);
-        const long sjp = ((long)k * ny + (j + 1 == ny ? 0 : j + 1)) * nx + i;
-        const long skp = ((long)(k + 1 == nz ? 0 : k + 1) * ny + j) * nx + i;
-        a[0] += (double)u[s];
-        a[1] += (double)u[4 * N + s];
-        a[2] += (double)u[N + s];
-        a[3] += (double)u[2 * N + s];
-        a[4] += (double)u[3 * N + s];
-        const double d = ((double)b[sip] - (double)b[s]) + ((double)b[N + sjp] - (double)b[N + s]) +
-                         ((double)b[2 * N + skp] - (double)b[2 * N + s]);
-        a[5] = fmax(a[5], fabs(d));
+    for (int r = blockIdx.x; r < g.ny * g.nz; r += gridDim.x) {
+        const int j = r % g.ny, k = r / g.ny;
+        const long row = (long)j * g.sy + (long)k * g.sz;
+        for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
+            const long s = row + i;
+            const long sip = row + (i + 1 == g.nx ? 0 : i + 1);
+            const long sjp = (long)(j + 1 == g.ny ? 0 : j + 1) * g.sy + (long)k * g.sz + i;
+            const long skp = (long)j * g.sy + (long)(k + 1 == g.nz ? 0 : k + 1) * g.sz + i;
+            a[0] += (double)u[s];
+            a[1] += (double)u[4 * N + s];
+            a[2] += (double)u[N + s];
+            a[3] += (double)u[2 * N + s];
+            a[4] += (double)u[3 * N + s];
+            const double d = ((double)b[sip] - (double)b[s]) + ((double)b[N + sjp] - (double)b[N + s]) +
+                             ((double)b[2 * N + skp] - (double)b[2 * N + s]);
+            a[5] = fmax(a[5], fabs(d));
+        }
     }
     __shared__ double red[6][256];
     for (int q = 0; q < 6; q++) red[q][threadIdx.x] = a[q];
@@ -631,21 +637,21 @@ __global__ void __launch_bounds__(256) monitor_soa_kernel(const T *__restrict__ 
     if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// AoS <-> SoA (C components): soa[c*N + s] = aos[s*C + c]
+// AoS (logical cell order, C per cell) <-> SoA planes on the padded grid
 template <typename T, int C>
-__global__ void __launch_bounds__(256) aos_to_soa_kernel(T *__restrict__ soa, const T *__restrict__ aos, long N)
+__global__ void __launch_bounds__(256) aos_soa_kernel(T *__restrict__ soa, T *__restrict__ aos, SoaGrid g, int to_soa)
 {
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < N * C; e += (long)gridDim.x * blockDim.x) {
-        const long c = e / N, s = e - c * N;
-        soa[e] = aos[s * C + c];
-    }
-}
-template <typename T, int C>
-__global__ void __launch_bounds__(256) soa_to_aos_kernel(T *__restrict__ aos, const T *__restrict__ soa, long N)
-{
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < N * C; e += (long)gridDim.x * blockDim.x) {
-        const long s = e / C, c = e - s * C;
-        aos[e] = soa[c * N + s];
+    for (int r = blockIdx.x; r < g.ny * g.nz; r += gridDim.x) {
+        const int j = r % g.ny, k = r / g.ny;
+        const long row = (long)j * g.sy + (long)k * g.sz;
+        const long lrow = (long)r * g.nx;
+        for (int e = threadIdx.x; e < g.nx * C; e += blockDim.x) {
+            const int i = e / C, c = e - i * C;
+            if (to_soa)
+                soa[c * g.N + row + i] = aos[(lrow + i) * C + c];
+            else
+                aos[(lrow + i) * C + c] = soa[c * g.N + row + i];
+        }
     }
 }
 
@@ -659,18 +665,37 @@ FastDiv make_fastdiv(int d)
     return r;
 }
 
-template <typename T> SoaBox make_soa_box(int fr, int nx, int ny, int nz)
+SoaGrid make_grid(int nx, int ny, int nz, int pad_y, int pad_z)
+{
+    SoaGrid g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.sy = nx + pad_y;
+    g.sz = g.sy * ny + pad_z;
+    g.N = g.sz * nz;
+    return g;
+}
+
+template <typename T> SoaBox make_soa_box(int fr, const SoaGrid &g)
 {
     SoaBox bx;
-    bx.f = make_frame(fr, nx, ny, nz);
-    bx.N = (long)nx * ny * nz;
+    Frame &f = bx.f;
+    if (fr == 0) {
+        f.ni = g.nx; f.nj = g.ny; f.nk = g.nz; f.si = 1; f.sj = g.sy; f.sk = g.sz;
+    } else if (fr == 1) {
+        f.ni = g.ny; f.nj = g.nx; f.nk = g.nz; f.si = g.sy; f.sj = 1; f.sk = g.sz;
+    } else {
+        f.ni = g.nz; f.nj = g.nx; f.nk = g.ny; f.si = g.sz; f.sj = 1; f.sk = g.sy;
+    }
+    bx.N = g.N;
     constexpr int VE = 16 / sizeof(T);
-    bx.n_div = make_fastdiv(bx.f.ni);
-    bx.n4_div = make_fastdiv(bx.f.ni / VE > 0 ? bx.f.ni / VE : 1);
-    // vectors need every plane and row start on a 16-byte boundary: nx a
-    // multiple of the vector width gives that for all strides; the x frame
-    // also needs whole vectors along the pencil
-    bx.vec_ok = (nx % VE == 0) && (bx.f.ni % VE == 0);
+    bx.n_div = make_fastdiv(f.ni);
+    bx.n4_div = make_fastdiv(f.ni / VE > 0 ? f.ni / VE : 1);
+    // 16-byte vectors need every row, plane and tile start on a vector
+    // boundary (nx, both strides and the plane size multiples of the vector
+    // width) and, in the x frame, whole vectors along the pencil
+    bx.vec_ok = (g.nx % VE == 0) && (g.sy % VE == 0) && (g.sz % VE == 0) && (g.N % VE == 0) && (f.ni % VE == 0);
     return bx;
 }
 
@@ -728,9 +753,9 @@ static hipError_t advect_soa_k(const T *u, const T *bin, T *bout, const SoaBox &
 }
 
 template <typename T>
-hipError_t launch_fluidx_soa(T *u, const T *b, int fr, int nx, int ny, int nz, const T *dt, hipStream_t st)
+hipError_t launch_fluidx_soa(T *u, const T *b, int fr, const SoaGrid &g, const T *dt, hipStream_t st)
 {
-    const SoaBox bx = make_soa_box<T>(fr, nx, ny, nz);
+    const SoaBox bx = make_soa_box<T>(fr, g);
     const int K = pick_k(bx.f.ni);
 #define SF0(KK) fluidx_soa_k<T, KK, 0>(u, b, bx, dt, st)
 #define SF1(KK) fluidx_soa_k<T, KK, 1>(u, b, bx, dt, st)
@@ -744,10 +769,10 @@ hipError_t launch_fluidx_soa(T *u, const T *b, int fr, int nx, int ny, int nz, c
 }
 
 template <typename T>
-hipError_t launch_advect_soa(const T *u, const T *bin, T *bout, int fr, int nx, int ny, int nz, const T *dt,
+hipError_t launch_advect_soa(const T *u, const T *bin, T *bout, int fr, const SoaGrid &g, const T *dt,
                              hipStream_t st)
 {
-    const SoaBox bx = make_soa_box<T>(fr, nx, ny, nz);
+    const SoaBox bx = make_soa_box<T>(fr, g);
     const int K = pick_k(bx.f.ni);
 #define SA0(KK) advect_soa_k<T, KK, 0>(u, bin, bout, bx, dt, st)
 #define SA1(KK) advect_soa_k<T, KK, 1>(u, bin, bout, bx, dt, st)
@@ -789,56 +814,50 @@ static int grid_for(long n)
     return (int)(nb < 4096 ? nb : 4096);
 }
 
-template <typename T>
-hipError_t launch_cfl_soa(const T *u, const T *b, int nx, int ny, int nz, int avg, unsigned long long *cmax,
-                          hipStream_t st)
+static int row_grid(const SoaGrid &g)
 {
-    const long N = (long)nx * ny * nz;
+    const int rows = g.ny * g.nz;
+    return rows < 8192 ? rows : 8192;
+}
+
+template <typename T>
+hipError_t launch_cfl_soa(const T *u, const T *b, const SoaGrid &g, int avg, unsigned long long *cmax, hipStream_t st)
+{
     if (avg)
-        hipLaunchKernelGGL((cfl_soa_kernel<T, true>), dim3(grid_for(N)), dim3(256), 0, st, u, b, nx, ny, nz, N, cmax);
+        hipLaunchKernelGGL((cfl_soa_kernel<T, true>), dim3(row_grid(g)), dim3(256), 0, st, u, b, g, cmax);
     else
-        hipLaunchKernelGGL((cfl_soa_kernel<T, false>), dim3(grid_for(N)), dim3(256), 0, st, u, b, nx, ny, nz, N,
-                           cmax);
+        hipLaunchKernelGGL((cfl_soa_kernel<T, false>), dim3(row_grid(g)), dim3(256), 0, st, u, b, g, cmax);
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_monitors_soa(const T *u, const T *b, int nx, int ny, int nz, double *part, int nblk, double *out,
+hipError_t launch_monitors_soa(const T *u, const T *b, const SoaGrid &g, double *part, int nblk, double *out,
                                hipStream_t st)
 {
-    const long N = (long)nx * ny * nz;
-    hipLaunchKernelGGL(monitor_soa_kernel<T>, dim3(nblk), dim3(256), 0, st, u, b, nx, ny, nz, N, part);
+    hipLaunchKernelGGL(monitor_soa_kernel<T>, dim3(nblk), dim3(256), 0, st, u, b, g, part);
     return launch_monitor_final(part, nblk, out, st);
 }
 
-template <typename T>
-hipError_t launch_aos_soa(T *dst, const T *src, int ncomp, long N, int to_soa, hipStream_t st)
+template <typename T> hipError_t launch_aos_soa(T *soa, T *aos, int ncomp, const SoaGrid &g, int to_soa, hipStream_t st)
 {
-    const int g = grid_for(N * ncomp);
-    if (ncomp == 5) {
-        if (to_soa)
-            hipLaunchKernelGGL((aos_to_soa_kernel<T, 5>), dim3(g), dim3(256), 0, st, dst, src, N);
-        else
-            hipLaunchKernelGGL((soa_to_aos_kernel<T, 5>), dim3(g), dim3(256), 0, st, dst, src, N);
-    } else {
-        if (to_soa)
-            hipLaunchKernelGGL((aos_to_soa_kernel<T, 3>), dim3(g), dim3(256), 0, st, dst, src, N);
-        else
-            hipLaunchKernelGGL((soa_to_aos_kernel<T, 3>), dim3(g), dim3(256), 0, st, dst, src, N);
-    }
+    if (ncomp == 5)
+        hipLaunchKernelGGL((aos_soa_kernel<T, 5>), dim3(row_grid(g)), dim3(256), 0, st, soa, aos, g, to_soa);
+    else
+        hipLaunchKernelGGL((aos_soa_kernel<T, 3>), dim3(row_grid(g)), dim3(256), 0, st, soa, aos, g, to_soa);
     return hipGetLastError();
 }
 
 #define SOA_INST(T)                                                                                                  \
-    template SoaBox make_soa_box<T>(int, int, int, int);                                                            \
-    template hipError_t launch_fluidx_soa<T>(T *, const T *, int, int, int, int, const T *, hipStream_t);          \
-    template hipError_t launch_advect_soa<T>(const T *, const T *, T *, int, int, int, int, const T *, hipStream_t); \
+    template SoaBox make_soa_box<T>(int, const SoaGrid &);                                                          \
+    template hipError_t launch_fluidx_soa<T>(T *, const T *, int, const SoaGrid &, const T *, hipStream_t);        \
+    template hipError_t launch_advect_soa<T>(const T *, const T *, T *, int, const SoaGrid &, const T *,           \
+                                             hipStream_t);                                                          \
     template hipError_t prepare_soa<T>(int, int, int);                                                              \
-    template hipError_t launch_cfl_soa<T>(const T *, const T *, int, int, int, int, unsigned long long *,          \
+    template hipError_t launch_cfl_soa<T>(const T *, const T *, const SoaGrid &, int, unsigned long long *,        \
                                           hipStream_t);                                                             \
-    template hipError_t launch_monitors_soa<T>(const T *, const T *, int, int, int, double *, int, double *,       \
+    template hipError_t launch_monitors_soa<T>(const T *, const T *, const SoaGrid &, double *, int, double *,     \
                                                hipStream_t);                                                        \
-    template hipError_t launch_aos_soa<T>(T *, const T *, int, long, int, hipStream_t);
+    template hipError_t launch_aos_soa<T>(T *, T *, int, const SoaGrid &, int, hipStream_t);
 SOA_INST(float)
 SOA_INST(double)
 

@@ -63,6 +63,8 @@ SIGNATURES = {
     "mhd_ctx_time": (_I, [_P, _DP, _DP]),
     "mhd_ctx_monitors": (_I, [_P, _DP]),
     "mhd_ctx_device_state": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
+    "mhd_ctx_layout": (_I, [_P, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long),
+                            ctypes.POINTER(ctypes.c_long)]),
     "mhd_ctx_profile_step": (_I, [_P, _I, _FP]),
 }
 
