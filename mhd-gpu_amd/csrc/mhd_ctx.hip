@@ -58,8 +58,10 @@ static int env_pad(const char *name, int dflt, int prec)
     const int ve = 16 / prec;
     return p < 0 ? 0 : (p + ve - 1) / ve * ve;
 }
+namespace mhd {
 int soa_pad_y(int prec) { return env_pad("MHD_PAD_Y", 0, prec); }
 int soa_pad_z(int prec) { return env_pad("MHD_PAD_Z", 256 / prec, prec); }
+}  // namespace mhd
 
 template <typename T> static hipError_t enqueue_cfl_dt(mhd_ctx *c)
 {
