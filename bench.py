@@ -70,6 +70,46 @@ def cpu_baseline(u0, b0, n, seconds_hint=30.0):
             "sample": f"1 step of the {n}^3 workload (same initial state), {dt:.1f} s wall"}
 
 
+def bench_batched(mhdgpu, steps=100, nbox=64, seed=3):
+    """The batched workload: nbox independent boxes, N uniform over
+    {32,48,64,96,128} (seeded), each the linear Alfven wave with its own seeded
+    phase and amplitude, `steps` steps each.  Compared with the same boxes run
+    one after the other through the plain path (one context per box)."""
+    from mhdgpu.ics import batch_box, batch_sizes
+    sizes = batch_sizes(nbox, seed)
+    states = [batch_box(n, seed=1000 + i) for i, n in enumerate(sizes)]
+    cells = sum(n ** 3 for n in sizes)
+    log(f"bench: batched, {nbox} boxes, {cells / 1e6:.1f} M cells")
+    # sequential plain runs (graph captured before each timed region)
+    seq_ms, ref_states = 0.0, {}
+    for i, n in enumerate(sizes):
+        sim = mhdgpu.Simulation(n, n, n)
+        sim.set_state(*states[i])
+        seq_ms += sim.step_timed(steps)
+        if i < 3:
+            ref_states[i] = sim.get_state()
+        sim.close()
+    # batched: autotune on the batch itself (state restored), then time
+    bt = mhdgpu.Batch([(n, n, n) for n in sizes])
+    for i in range(nbox):
+        bt.set_state(i, *states[i])
+    tuned = bt.autotune(trial_steps=2)
+    t0 = time.perf_counter()
+    bat_ms = bt.step_timed(steps)
+    mhdgpu.check(mhdgpu.lib().mhd_device_sync())
+    wall = time.perf_counter() - t0
+    same = all(np.array_equal(bt.get_state(i)[0], ref_states[i][0]) and
+               np.array_equal(bt.get_state(i)[1], ref_states[i][1]) for i in ref_states)
+    bt.close()
+    zu = cells * steps
+    return {"boxes": nbox, "sizes_hist": {str(k): sizes.count(k) for k in sorted(set(sizes))},
+            "cells": cells, "steps": steps,
+            "batched_zone_updates_per_s": zu / (bat_ms * 1e-3), "batched_ms": bat_ms, "batched_wall_ms": wall * 1e3,
+            "sequential_zone_updates_per_s": zu / (seq_ms * 1e-3), "sequential_ms": seq_ms,
+            "speedup_vs_sequential": seq_ms / bat_ms, "schedule": tuned,
+            "boxes_identical_to_plain": bool(same)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +117,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--mode", choices=["plain", "copy"], default="plain",
                     help="copy: time the plain copy kernel on 1 GiB (counter calibration)")
     args = ap.parse_args()
@@ -157,6 +198,8 @@ def main():
                       "sweeps": {f"{i:02d}_{s}": float(v) for i, (s, v) in enumerate(zip(SWEEPS, sweep_ms))}},
         "monitors_fp64": {"mass": m[0], "energy": m[1], "max_divB": m[5], "t": t, "t64": t64, "finite": finite},
     }
+    if not args.no_batched:
+        out["batched"] = bench_batched(mhdgpu)
     if not args.no_cpu:
         log("bench: CPU baseline")
         out["cpu_baseline"] = cpu_baseline(u0, b0, n)

@@ -155,6 +155,30 @@ int mhd_ctx_layout(mhd_ctx *ctx, long *sy, long *sz, long *plane);
  * advect z, fluidx z, advect y, fluidx y, advect x, fluidx x), averaged */
 int mhd_ctx_profile_step(mhd_ctx *ctx, int reps, float *ms_out13);
 
+/* ---- batched form: many independent boxes of uneven sizes ----------------
+ * No counterpart in the reference (one compile-time box per program,
+ * Heterogeneous_MHD/C/parameter.h:6).  Each box advances with its own dt
+ * exactly as a lone context would (results are bit-identical to
+ * mhd_ctx_step on the same box); one step of the batch is one CFL launch,
+ * one dt launch and 12 persistent sweep kernels over all boxes, captured in
+ * a hipGraph.  FP32.  dims holds nx,ny,nz of each box. */
+typedef struct mhd_batch mhd_batch;
+int mhd_batch_create(mhd_batch **bt, int nbox, const int *dims, int avg_b);
+int mhd_batch_destroy(mhd_batch *bt);
+int mhd_batch_set_state(mhd_batch *bt, int box, const float *h_u, const float *h_b);
+int mhd_batch_get_state(mhd_batch *bt, int box, float *h_u, float *h_b);
+int mhd_batch_step(mhd_batch *bt, int nsteps);
+int mhd_batch_step_timed(mhd_batch *bt, int nsteps, float *ms);
+int mhd_batch_sync(mhd_batch *bt);
+int mhd_batch_dts(mhd_batch *bt, int box, double *h_dts, int max, int *count);
+/* schedule of the sweeps: blocks per CU of the persistent kernels (0 =
+ * occupancy limit) and policy (0 = persistent, one queue per XCD with
+ * stealing; 1 = persistent, one global queue; 2 = one block per work item) */
+int mhd_batch_set_schedule(mhd_batch *bt, int blocks_per_cu, int policy);
+/* times the candidate schedules on the batch itself (trial_steps each),
+ * restores every box's state and clock, keeps the fastest */
+int mhd_batch_autotune(mhd_batch *bt, int trial_steps, int *blocks_per_cu, int *policy, float *ms_per_step);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,96 @@
+// Batched engine: many independent boxes of uneven sizes advanced together,
+// each with its own dt.  One step is 1 CFL launch over all boxes, 1 dt
+// launch, and 12 sweeps, each ONE persistent kernel that pulls (box, tile)
+// work items from per-XCD queues (own queue first, then stealing).  The step
+// is captured in a hipGraph and replayed.  Boxes live in slots; the active
+// set can change between steps (the server admits and retires requests that
+// way).  FP32 only, the reference's precision.
+#pragma once
+#include <vector>
+
+#include "mhd_soa.h"
+
+namespace mhd {
+
+struct BoxDesc {
+    float *u, *b, *b2;   // SoA state and the second field buffer
+    SoaGrid g;
+    SoaBox fr[3];        // the three sweep frames
+    int K[3];            // cells per lane of each frame
+    int ntj[3];          // tiles along j of each frame
+};
+
+struct WorkItem {
+    int box, tile;
+};
+struct CflItem {
+    int box, row0, nrows;
+};
+
+class Engine {
+public:
+    // max_boxes slots, dt history of cap steps per slot
+    Engine(int max_boxes, int cap, int avg_b);
+    ~Engine();
+    int init();  // allocations; MHD_OK or an error code
+    hipStream_t stream() const { return st_; }
+
+    // slots: device state of one box (44 B per cell)
+    int add_box(int nx, int ny, int nz, int *slot);
+    int remove_box(int slot);
+    int set_state(int slot, const float *h_u, const float *h_b);
+    int get_state(int slot, float *h_u, float *h_b);
+    // async variants through a caller-provided pinned host buffer: the
+    // caller must keep h_u/h_b alive until the stream has passed them
+    int reset_clock(int slot, double tf);
+    int dts(int slot, double *h, int max, int *count);
+    int time(int slot, double *t, double *t64);
+    long cells(int slot) const;
+
+    // the set of slots a step advances (rebuilds work lists; re-captures)
+    int set_active(const std::vector<int> &slots);
+    const std::vector<int> &active() const { return active_; }
+    int step(int nsteps);
+    int step_timed(int nsteps, float *ms);
+    int sync();
+
+    // schedule: persistent blocks per CU (0 = occupancy limit) and policy
+    // (0: persistent, one queue per XCD + stealing; 1: persistent, one
+    // global queue; 2: one block per work item)
+    int set_schedule(int blocks_per_cu, int policy);
+    int autotune(int trial_steps, int *bpc, int *policy, float *best_ms);
+
+private:
+    int build();
+    int capture();
+    void drop_graph();
+    int enqueue_step();
+    int save_all(std::vector<float *> &saved);
+    int restore_all(std::vector<float *> &saved);
+
+    int max_boxes_, cap_, avg_;
+    hipStream_t st_ = nullptr;
+    std::vector<BoxDesc> boxes_;   // host copy, per slot
+    std::vector<bool> used_;
+    std::vector<int> active_;
+    std::vector<double> tf_;
+    // device arrays
+    BoxDesc *d_boxes_ = nullptr;
+    float *d_dt_ = nullptr, *d_t_ = nullptr, *d_tf_ = nullptr, *d_dts_ = nullptr;
+    double *d_t64_ = nullptr;
+    unsigned long long *d_cmax_ = nullptr;
+    int *d_iter_ = nullptr, *d_active_ = nullptr;
+    unsigned *d_ctr_ = nullptr;        // 12 sweeps x 8 queues
+    int *d_part_ = nullptr;            // 3 frames x 9 partition bounds
+    WorkItem *d_items_[3] = {nullptr, nullptr, nullptr};
+    CflItem *d_cfl_ = nullptr;
+    int nitems_[3] = {0, 0, 0}, ncfl_ = 0, cap_items_[3] = {0, 0, 0}, cap_cfl_ = 0;
+    int kmax_ = 1, lds_fluidx_ = 0, lds_advect_ = 0;
+    int bpc_ = 0, policy_ = 0, grid_ = 0, ncu_ = 0;
+    bool dirty_ = true;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+};
+
+}  // namespace mhd

@@ -1,1 +1,928 @@
-// placeholder
+// Batched form: the engine of mhd_batch.h and the mhd_batch_* entry points.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "../../include/mhd_api.h"
+#include "mhd_batch.h"
+#include "mhd_internal.h"
+#include "mhd_soa_tiles.h"
+
+namespace mhd {
+
+constexpr int kBW = 4;       // pencils per tile (one float4 row)
+constexpr int kQueues = 8;   // one work queue per XCD
+static const int kSweepFrameB[12] = {0, 0, 1, 1, 2, 2, 2, 2, 1, 1, 0, 0};
+static const int kSweepOpB[12] = {0, 1, 0, 1, 0, 1, 1, 0, 1, 0, 1, 0};  // 0 fluidx, 1 advectbyzx
+
+// the field is in b before sweep 0; each advectbyzx moves it to the other buffer
+static int flip_before(int s)
+{
+    int n = 0;
+    for (int i = 0; i < s; i++) n += kSweepOpB[i];
+    return n & 1;
+}
+
+// ---------------------------------------------------------------------------
+// persistent sweep kernel: blocks pull (box, tile) items from the queue of
+// their XCD (blocks are dealt round-robin over the 8 XCDs), then steal from
+// the other queues; the item ranges were cut so that each XCD's range is
+// about equal in work and contiguous in memory.
+// ---------------------------------------------------------------------------
+// One work item, as a real call: inlined into the persistent loop, the tile
+// body with its descriptor loaded per iteration hits a register-class error
+// in the gfx950 backend (ROCm 7.2); a call costs a few hundred cycles of a
+// tile's tens of thousands.
+template <int FR, int KMAX, bool ADV>
+__device__ __noinline__ void batch_item(const BoxDesc *__restrict__ boxes, WorkItem wi, const float *__restrict__ dt,
+                                        int flip, float *smem)
+{
+    const BoxDesc &bd = boxes[wi.box];
+    const int tile = wi.tile;
+    const float *dtp = dt + wi.box;
+    float *const u = bd.u;
+    const float *bcur = flip ? bd.b2 : bd.b;
+    float *bnext = flip ? bd.b : bd.b2;
+    const SoaBox bx = bd.fr[FR];  // by value: the tile reads it many times
+    const int ntj = bd.ntj[FR];
+#define MHD_BITEM(KK)                                                                                                \
+    if constexpr (ADV)                                                                                               \
+        advect_soa_tile<float, KK, kBW, FR>(u, bcur, bnext, bx, dtp, tile, ntj, smem);                              \
+    else                                                                                                             \
+        fluidx_soa_tile<float, KK, kBW, FR>(u, bcur, bx, dtp, tile, ntj, smem);
+    switch (bd.K[FR]) {
+    case 1: { MHD_BITEM(1) } break;
+    case 2: if constexpr (KMAX >= 2) { MHD_BITEM(2) } break;
+    case 3: if constexpr (KMAX >= 4) { MHD_BITEM(3) } break;
+    case 4: if constexpr (KMAX >= 4) { MHD_BITEM(4) } break;
+    case 5: if constexpr (KMAX >= 8) { MHD_BITEM(5) } break;
+    case 6: if constexpr (KMAX >= 8) { MHD_BITEM(6) } break;
+    case 8: if constexpr (KMAX >= 8) { MHD_BITEM(8) } break;
+    default: break;
+    }
+#undef MHD_BITEM
+}
+
+template <int FR, int KMAX, bool ADV>
+__global__ void __launch_bounds__(64 * kBW)
+    batch_sweep_kernel(const BoxDesc *__restrict__ boxes, const WorkItem *__restrict__ items,
+                       const int *__restrict__ part, unsigned *__restrict__ ctr, const float *__restrict__ dt, int flip,
+                       int nq)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float *smem = reinterpret_cast<float *>(smem_raw);
+    __shared__ int s_it;
+    const int q0 = nq > 1 ? (int)(blockIdx.x % nq) : 0;
+    int tried = 0;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            int it = -1;
+            while (tried < nq) {
+                const int q = (q0 + tried) % nq;
+                const int cand = part[q] + (int)atomicAdd(&ctr[q], 1u);
+                if (cand < part[q + 1]) {
+                    it = cand;
+                    break;
+                }
+                tried++;
+            }
+            s_it = it;
+        }
+        __syncthreads();
+        const int it = s_it;
+        __syncthreads();
+        if (it < 0) break;
+        batch_item<FR, KMAX, ADV>(boxes, items[it], dt, flip, smem);
+        __syncthreads();  // the LDS tile is reused by the next item
+    }
+}
+
+// The same sweep with one block per work item (no loop, tile body inlined):
+// the hardware dispatcher balances; blocks map to items so that each XCD
+// receives a contiguous range of the largest-first item list.
+template <int FR, int KMAX, bool ADV>
+__global__ void __launch_bounds__(64 * kBW)
+    batch_grid_kernel(const BoxDesc *__restrict__ boxes, const WorkItem *__restrict__ items,
+                      const float *__restrict__ dt, int flip)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float *smem = reinterpret_cast<float *>(smem_raw);
+    const WorkItem wi = items[xcd_tile(blockIdx.x, gridDim.x)];
+    const BoxDesc &bd = boxes[wi.box];
+    float *const u = bd.u;
+    const float *bcur = flip ? bd.b2 : bd.b;
+    float *bnext = flip ? bd.b : bd.b2;
+    const float *dtp = dt + wi.box;
+#define MHD_GITEM(KK)                                                                                                \
+    if constexpr (ADV)                                                                                               \
+        advect_soa_tile<float, KK, kBW, FR>(u, bcur, bnext, bd.fr[FR], dtp, wi.tile, bd.ntj[FR], smem);             \
+    else                                                                                                             \
+        fluidx_soa_tile<float, KK, kBW, FR>(u, bcur, bd.fr[FR], dtp, wi.tile, bd.ntj[FR], smem);
+    switch (bd.K[FR]) {
+    case 1: { MHD_GITEM(1) } break;
+    case 2: if constexpr (KMAX >= 2) { MHD_GITEM(2) } break;
+    case 3: if constexpr (KMAX >= 4) { MHD_GITEM(3) } break;
+    case 4: if constexpr (KMAX >= 4) { MHD_GITEM(4) } break;
+    case 5: if constexpr (KMAX >= 8) { MHD_GITEM(5) } break;
+    case 6: if constexpr (KMAX >= 8) { MHD_GITEM(6) } break;
+    case 8: if constexpr (KMAX >= 8) { MHD_GITEM(8) } break;
+    default: break;
+    }
+#undef MHD_GITEM
+}
+
+// CFL over the active boxes: one block per (box, row range); per-box max
+template <bool AVG>
+__global__ void __launch_bounds__(256) batch_cfl_kernel(const BoxDesc *__restrict__ boxes,
+                                                        const CflItem *__restrict__ items,
+                                                        unsigned long long *__restrict__ cmax)
+{
+    const CflItem it = items[blockIdx.x];
+    const BoxDesc &bd = boxes[it.box];
+    const SoaGrid g = bd.g;
+    const float *u = bd.u, *b = bd.b;
+    const long N = g.N;
+    float c = 0.f;
+    for (int r = it.row0; r < it.row0 + it.nrows; r++) {
+        const int j = r % g.ny, k = r / g.ny;
+        const long row = (long)j * g.sy + (long)k * g.sz;
+        for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
+            const long s = row + i;
+            const float b1 = b[s], b2 = b[N + s], b3 = b[2 * N + s];
+            float B;
+            if constexpr (AVG) {
+                const long sip = row + (i + 1 == g.nx ? 0 : i + 1);
+                const long sjp = (long)(j + 1 == g.ny ? 0 : j + 1) * g.sy + (long)k * g.sz + i;
+                const long skp = (long)j * g.sy + (long)(k + 1 == g.nz ? 0 : k + 1) * g.sz + i;
+                const float bxa = (float)((double)(b1 + b[sip]) / 2.0);
+                const float bya = (float)((double)(b2 + b[N + sjp]) / 2.0);
+                const float bza = (float)((double)(b3 + b[2 * N + skp]) / 2.0);
+                B = bxa * bxa + bya * bya + bza * bza;
+            } else {
+                B = b1 * b1 + b2 * b2 + b3 * b3;
+            }
+            const float sp = cfl_speed<float>(u[s], u[N + s], u[2 * N + s], u[3 * N + s], u[4 * N + s], B);
+            c = c > sp ? c : sp;
+        }
+    }
+    c = wave_max(c);
+    __shared__ float red[4];
+    if (lane_id() == 0) red[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = red[0];
+        for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = fmaxf(m, red[i]);
+        atomicMax(cmax + it.box, (unsigned long long)__float_as_uint(m));
+    }
+}
+
+// per-box dt (Heterogeneous_MHD/C/mhd.c:63-65) for the active slots; clears
+// the CFL maxima and the sweep queues for this step
+__global__ void batch_dt_kernel(const int *__restrict__ active, int nactive, unsigned long long *cmax, float *dt,
+                                float *t, double *t64, const float *tf, float *dts, int cap, int *iter, unsigned *ctr,
+                                int nctr)
+{
+    for (int a = threadIdx.x; a < nactive; a += blockDim.x) {
+        const int bx = active[a];
+        const float c = __uint_as_float((unsigned)cmax[bx]);
+        cmax[bx] = 0ull;
+        const float cfl = (float)(1.0 / (double)c);
+        float d = (float)(0.9 * (double)cfl);
+        if (d > (tf[bx] - t[bx]) / 2.f) d = (tf[bx] - t[bx]) / 2.f;
+        t[bx] = t[bx] + 2.f * d;
+        t64[bx] = t64[bx] + 2.0 * (double)d;
+        dt[bx] = d;
+        const int it = iter[bx];
+        if (it < cap) dts[(long)bx * cap + it] = d;
+        iter[bx] = it + 1;
+    }
+    for (int i = threadIdx.x; i < nctr; i += blockDim.x) ctr[i] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// host: launch helpers
+// ---------------------------------------------------------------------------
+template <int FR, int KMAX, bool ADV> static const void *kern_ptr()
+{
+    return (const void *)batch_sweep_kernel<FR, KMAX, ADV>;
+}
+
+template <int FR, int KMAX, bool ADV> static const void *grid_ptr()
+{
+    return (const void *)batch_grid_kernel<FR, KMAX, ADV>;
+}
+
+static const void *grid_kernel(int fr, int kmax, bool adv)
+{
+#define MHD_GP(F)                                                                                                    \
+    if (fr == F) {                                                                                                   \
+        if (kmax <= 2) return adv ? grid_ptr<F, 2, true>() : grid_ptr<F, 2, false>();                              \
+        if (kmax <= 4) return adv ? grid_ptr<F, 4, true>() : grid_ptr<F, 4, false>();                              \
+        return adv ? grid_ptr<F, 8, true>() : grid_ptr<F, 8, false>();                                              \
+    }
+    MHD_GP(0)
+    MHD_GP(1)
+    MHD_GP(2)
+#undef MHD_GP
+    return nullptr;
+}
+
+static hipError_t launch_grid(int fr, int kmax, bool adv, int nitems, size_t lds, hipStream_t st,
+                              const BoxDesc *boxes, const WorkItem *items, const float *dt, int flip)
+{
+#define MHD_LG(F, KM, A)                                                                                             \
+    hipLaunchKernelGGL((batch_grid_kernel<F, KM, A>), dim3(nitems), dim3(64 * kBW), lds, st, boxes, items, dt, flip)
+#define MHD_LGF(F)                                                                                                   \
+    if (fr == F) {                                                                                                   \
+        if (kmax <= 2) {                                                                                             \
+            if (adv) MHD_LG(F, 2, true); else MHD_LG(F, 2, false);                                                   \
+        } else if (kmax <= 4) {                                                                                      \
+            if (adv) MHD_LG(F, 4, true); else MHD_LG(F, 4, false);                                                   \
+        } else {                                                                                                     \
+            if (adv) MHD_LG(F, 8, true); else MHD_LG(F, 8, false);                                                   \
+        }                                                                                                            \
+    }
+    MHD_LGF(0)
+    MHD_LGF(1)
+    MHD_LGF(2)
+#undef MHD_LGF
+#undef MHD_LG
+    return hipGetLastError();
+}
+
+static const void *sweep_kernel(int fr, int kmax, bool adv)
+{
+#define MHD_KP(F)                                                                                                    \
+    if (fr == F) {                                                                                                   \
+        if (kmax <= 2) return adv ? kern_ptr<F, 2, true>() : kern_ptr<F, 2, false>();                              \
+        if (kmax <= 4) return adv ? kern_ptr<F, 4, true>() : kern_ptr<F, 4, false>();                              \
+        return adv ? kern_ptr<F, 8, true>() : kern_ptr<F, 8, false>();                                              \
+    }
+    MHD_KP(0)
+    MHD_KP(1)
+    MHD_KP(2)
+#undef MHD_KP
+    return nullptr;
+}
+
+static hipError_t launch_sweep(int fr, int kmax, bool adv, int grid, size_t lds, hipStream_t st, const BoxDesc *boxes,
+                               const WorkItem *items, const int *part, unsigned *ctr, const float *dt, int flip, int nq)
+{
+#define MHD_LS(F, KM, A)                                                                                             \
+    hipLaunchKernelGGL((batch_sweep_kernel<F, KM, A>), dim3(grid), dim3(64 * kBW), lds, st, boxes, items, part, ctr,  \
+                       dt, flip, nq)
+#define MHD_LF(F)                                                                                                    \
+    if (fr == F) {                                                                                                   \
+        if (kmax <= 2) {                                                                                             \
+            if (adv) MHD_LS(F, 2, true); else MHD_LS(F, 2, false);                                                   \
+        } else if (kmax <= 4) {                                                                                      \
+            if (adv) MHD_LS(F, 4, true); else MHD_LS(F, 4, false);                                                   \
+        } else {                                                                                                     \
+            if (adv) MHD_LS(F, 8, true); else MHD_LS(F, 8, false);                                                   \
+        }                                                                                                            \
+    }
+    MHD_LF(0)
+    MHD_LF(1)
+    MHD_LF(2)
+#undef MHD_LF
+#undef MHD_LS
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Engine
+// ---------------------------------------------------------------------------
+Engine::Engine(int max_boxes, int cap, int avg_b) : max_boxes_(max_boxes), cap_(cap), avg_(avg_b ? 1 : 0) {}
+
+Engine::~Engine()
+{
+    if (st_) (void)hipStreamSynchronize(st_);
+    drop_graph();
+    for (size_t s = 0; s < boxes_.size(); s++)
+        if (used_[s]) {
+            (void)hipFree(boxes_[s].u);
+            (void)hipFree(boxes_[s].b);
+            (void)hipFree(boxes_[s].b2);
+        }
+    void *ps[] = {d_boxes_, d_dt_, d_t_, d_tf_, d_dts_, d_t64_, d_cmax_, d_iter_, d_active_, d_ctr_, d_part_,
+                  d_items_[0], d_items_[1], d_items_[2], d_cfl_};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+    if (st_) (void)hipStreamDestroy(st_);
+}
+
+int Engine::init()
+{
+    if (max_boxes_ < 1 || cap_ < 1) return fail(MHD_ERR_ARG, "engine: bad capacity");
+    hipError_t e = hipStreamCreate(&st_);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+    const size_t B = max_boxes_;
+    struct A {
+        void **p;
+        size_t n;
+    } al[] = {{(void **)&d_boxes_, B * sizeof(BoxDesc)}, {(void **)&d_dt_, B * 4},   {(void **)&d_t_, B * 4},
+              {(void **)&d_tf_, B * 4},                   {(void **)&d_dts_, B * cap_ * 4},
+              {(void **)&d_t64_, B * 8},                  {(void **)&d_cmax_, B * 8}, {(void **)&d_iter_, B * 4},
+              {(void **)&d_active_, B * 4},               {(void **)&d_ctr_, 12 * kQueues * 4},
+              {(void **)&d_part_, 3 * (kQueues + 1) * 4}};
+    for (auto &a : al) {
+        e = hipMalloc(a.p, a.n);
+        if (e != hipSuccess) return fail(MHD_ERR_NOMEM, "engine: hipMalloc(%zu): %s", a.n, hipGetErrorString(e));
+    }
+    e = hipMemsetAsync(d_cmax_, 0, B * 8, st_);
+    if (e == hipSuccess) e = hipMemsetAsync(d_ctr_, 0, 12 * kQueues * 4, st_);
+    if (e == hipSuccess) e = hipEventCreate(&ev0_);
+    if (e == hipSuccess) e = hipEventCreate(&ev1_);
+    if (e != hipSuccess) return hip_fail(e, "engine init");
+    boxes_.assign(B, BoxDesc());
+    used_.assign(B, false);
+    tf_.assign(B, INFINITY);
+    hipDeviceProp_t prop;
+    int dev = 0;
+    e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+    ncu_ = prop.multiProcessorCount;
+    for (int fr = 0; fr < 3; fr++)
+        for (int km : {2, 4, 8})
+            for (int adv = 0; adv < 2; adv++) {
+                // the largest tile (n = 512) needs 76 KB; the persistent
+                // kernels also hold a little static LDS, so stay below 160 KB
+                e = hipFuncSetAttribute(sweep_kernel(fr, km, adv), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        128 * 1024);
+                if (e == hipSuccess)
+                    e = hipFuncSetAttribute(grid_kernel(fr, km, adv), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            128 * 1024);
+                if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute");
+            }
+    e = hipStreamSynchronize(st_);
+    return e == hipSuccess ? MHD_OK : hip_fail(e, "engine init");
+}
+
+long Engine::cells(int slot) const
+{
+    const SoaGrid &g = boxes_[slot].g;
+    return (long)g.nx * g.ny * g.nz;
+}
+
+int Engine::add_box(int nx, int ny, int nz, int *slot)
+{
+    int rc = check_shape3(nx, ny, nz);
+    if (rc) return rc;
+    int s = -1;
+    for (int i = 0; i < max_boxes_; i++)
+        if (!used_[i]) {
+            s = i;
+            break;
+        }
+    if (s < 0) return fail(MHD_ERR_FULL, "engine: all %d slots in use", max_boxes_);
+    BoxDesc d;
+    memset(&d, 0, sizeof d);
+    d.g = make_grid(nx, ny, nz, soa_pad_y(4), soa_pad_z(4));
+    const size_t N = (size_t)d.g.N;
+    hipError_t e = hipMalloc((void **)&d.u, 5 * N * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&d.b, 3 * N * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&d.b2, 3 * N * 4);
+    if (e != hipSuccess) {
+        if (d.u) (void)hipFree(d.u);
+        if (d.b) (void)hipFree(d.b);
+        if (d.b2) (void)hipFree(d.b2);
+        return fail(MHD_ERR_NOMEM, "engine: box %dx%dx%d: %s", nx, ny, nz, hipGetErrorString(e));
+    }
+    for (int fr = 0; fr < 3; fr++) {
+        d.fr[fr] = make_soa_box<float>(fr, d.g);
+        d.K[fr] = pick_k(d.fr[fr].f.ni);
+        d.ntj[fr] = (d.fr[fr].f.nj + kBW - 1) / kBW;
+    }
+    boxes_[s] = d;
+    used_[s] = true;
+    e = hipMemcpyAsync(d_boxes_ + s, &boxes_[s], sizeof(BoxDesc), hipMemcpyHostToDevice, st_);
+    if (e == hipSuccess) e = hipStreamSynchronize(st_);
+    if (e != hipSuccess) return hip_fail(e, "engine add_box");
+    rc = reset_clock(s, INFINITY);
+    if (rc) return rc;
+    *slot = s;
+    return MHD_OK;
+}
+
+int Engine::remove_box(int slot)
+{
+    if (slot < 0 || slot >= max_boxes_ || !used_[slot]) return fail(MHD_ERR_ARG, "engine: bad slot %d", slot);
+    hipError_t e = hipStreamSynchronize(st_);
+    if (e != hipSuccess) return hip_fail(e, "engine remove_box");
+    auto it = std::find(active_.begin(), active_.end(), slot);
+    if (it != active_.end()) {
+        active_.erase(it);
+        dirty_ = true;
+    }
+    (void)hipFree(boxes_[slot].u);
+    (void)hipFree(boxes_[slot].b);
+    (void)hipFree(boxes_[slot].b2);
+    memset(&boxes_[slot], 0, sizeof(BoxDesc));
+    used_[slot] = false;
+    return MHD_OK;
+}
+
+static int transfer(BoxDesc &d, void *h_u, void *h_b, int to_device, hipStream_t st)
+{
+    const long nc = (long)d.g.nx * d.g.ny * d.g.nz;
+    void *tmp = nullptr;
+    hipError_t e = hipMalloc(&tmp, 5 * nc * 4);
+    if (e != hipSuccess) return fail(MHD_ERR_NOMEM, "state transfer buffer: %s", hipGetErrorString(e));
+    struct P {
+        void *h;
+        float *d;
+        int nc;
+    } ps[2] = {{h_u, d.u, 5}, {h_b, d.b, 3}};
+    for (auto &p : ps) {
+        if (e != hipSuccess) break;
+        if (to_device) {
+            e = hipMemcpyAsync(tmp, p.h, (size_t)nc * p.nc * 4, hipMemcpyHostToDevice, st);
+            if (e == hipSuccess) e = launch_aos_soa<float>(p.d, (float *)tmp, p.nc, d.g, 1, st);
+        } else {
+            e = launch_aos_soa<float>(p.d, (float *)tmp, p.nc, d.g, 0, st);
+            if (e == hipSuccess) e = hipMemcpyAsync(p.h, tmp, (size_t)nc * p.nc * 4, hipMemcpyDeviceToHost, st);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+    }
+    (void)hipFree(tmp);
+    return e == hipSuccess ? MHD_OK : hip_fail(e, "state transfer");
+}
+
+int Engine::set_state(int slot, const float *h_u, const float *h_b)
+{
+    if (slot < 0 || slot >= max_boxes_ || !used_[slot] || !h_u || !h_b) return fail(MHD_ERR_ARG, "set_state");
+    hipError_t e = hipStreamSynchronize(st_);
+    if (e != hipSuccess) return hip_fail(e, "set_state");
+    return transfer(boxes_[slot], const_cast<float *>(h_u), const_cast<float *>(h_b), 1, st_);
+}
+
+int Engine::get_state(int slot, float *h_u, float *h_b)
+{
+    if (slot < 0 || slot >= max_boxes_ || !used_[slot] || !h_u || !h_b) return fail(MHD_ERR_ARG, "get_state");
+    return transfer(boxes_[slot], h_u, h_b, 0, st_);
+}
+
+int Engine::reset_clock(int slot, double tf)
+{
+    if (slot < 0 || slot >= max_boxes_ || !used_[slot]) return fail(MHD_ERR_ARG, "reset_clock: bad slot");
+    tf_[slot] = tf;
+    const float tff = (float)tf;
+    hipError_t e = hipMemcpyAsync(d_tf_ + slot, &tff, 4, hipMemcpyHostToDevice, st_);
+    if (e == hipSuccess) e = hipMemsetAsync(d_t_ + slot, 0, 4, st_);
+    if (e == hipSuccess) e = hipMemsetAsync(d_dt_ + slot, 0, 4, st_);
+    if (e == hipSuccess) e = hipMemsetAsync(d_t64_ + slot, 0, 8, st_);
+    if (e == hipSuccess) e = hipMemsetAsync(d_iter_ + slot, 0, 4, st_);
+    if (e == hipSuccess) e = hipMemsetAsync(d_cmax_ + slot, 0, 8, st_);
+    if (e == hipSuccess) e = hipStreamSynchronize(st_);
+    return e == hipSuccess ? MHD_OK : hip_fail(e, "reset_clock");
+}
+
+int Engine::dts(int slot, double *h, int max, int *count)
+{
+    if (slot < 0 || slot >= max_boxes_ || !used_[slot] || !count) return fail(MHD_ERR_ARG, "dts: bad argument");
+    int it = 0;
+    hipError_t e = hipStreamSynchronize(st_);
+    if (e == hipSuccess) e = hipMemcpy(&it, d_iter_ + slot, 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "dts");
+    *count = it;
+    int n = std::min(std::min(it, max), cap_);
+    if (n > 0 && h) {
+        std::vector<float> f(n);
+        e = hipMemcpy(f.data(), d_dts_ + (long)slot * cap_, n * 4, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(e, "dts copy");
+        for (int i = 0; i < n; i++) h[i] = f[i];
+    }
+    return MHD_OK;
+}
+
+int Engine::time(int slot, double *t, double *t64)
+{
+    if (slot < 0 || slot >= max_boxes_ || !used_[slot]) return fail(MHD_ERR_ARG, "time: bad slot");
+    float tf = 0;
+    double td = 0;
+    hipError_t e = hipStreamSynchronize(st_);
+    if (e == hipSuccess) e = hipMemcpy(&tf, d_t_ + slot, 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(&td, d_t64_ + slot, 8, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "time");
+    if (t) *t = tf;
+    if (t64) *t64 = td;
+    return MHD_OK;
+}
+
+int Engine::set_active(const std::vector<int> &slots)
+{
+    for (int s : slots)
+        if (s < 0 || s >= max_boxes_ || !used_[s]) return fail(MHD_ERR_ARG, "set_active: bad slot %d", s);
+    active_ = slots;
+    dirty_ = true;
+    return MHD_OK;
+}
+
+int Engine::set_schedule(int bpc, int policy)
+{
+    if (bpc < 0 || policy < 0 || policy > 2) return fail(MHD_ERR_ARG, "set_schedule: bad argument");
+    bpc_ = bpc;
+    policy_ = policy;
+    dirty_ = true;
+    return MHD_OK;
+}
+
+template <typename X> static hipError_t ensure_cap(X **p, int *cap, int need)
+{
+    if (need <= *cap) return hipSuccess;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    int c = std::max(need, 2 * *cap);
+    hipError_t e = hipMalloc((void **)p, sizeof(X) * (size_t)c);
+    *cap = e == hipSuccess ? c : 0;
+    return e;
+}
+
+int Engine::build()
+{
+    hipError_t e = hipStreamSynchronize(st_);
+    if (e != hipSuccess) return hip_fail(e, "engine build");
+    drop_graph();
+    // largest boxes first: the queues then hand out the longest items first
+    std::vector<int> order = active_;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cells(a) > cells(b); });
+    kmax_ = 1;
+    lds_fluidx_ = lds_advect_ = 0;
+    std::vector<WorkItem> items[3];
+    std::vector<int> part(3 * (kQueues + 1), 0);
+    for (int fr = 0; fr < 3; fr++) {
+        std::vector<double> cost;
+        for (int s : order) {
+            const BoxDesc &d = boxes_[s];
+            const int nt = d.ntj[fr] * d.fr[fr].f.nk;
+            kmax_ = std::max(kmax_, d.K[fr]);
+            lds_fluidx_ = std::max(lds_fluidx_, (int)(4 * (9 * kBW + 1) * d.fr[fr].f.ni));
+            lds_advect_ = std::max(lds_advect_, (int)(4 * (7 * kBW + 2) * d.fr[fr].f.ni));
+            for (int t = 0; t < nt; t++) {
+                items[fr].push_back({s, t});
+                cost.push_back((double)d.K[fr]);  // lane-slots per tile ~ K (W pencils x 64 lanes x K)
+            }
+        }
+        const int n = (int)items[fr].size();
+        int *pt = part.data() + fr * (kQueues + 1);
+        if (policy_ == 0) {
+            double tot = 0;
+            for (double c : cost) tot += c;
+            double acc = 0;
+            int q = 1;
+            pt[0] = 0;
+            for (int i = 0; i < n && q < kQueues; i++) {
+                acc += cost[i];
+                while (q < kQueues && acc >= tot * q / kQueues) pt[q++] = i + 1;
+            }
+            for (; q <= kQueues; q++) pt[q] = n;
+        } else {
+            pt[0] = 0;
+            for (int q = 1; q <= kQueues; q++) pt[q] = n;  // queue 0 holds everything
+        }
+        nitems_[fr] = n;
+        e = ensure_cap(&d_items_[fr], &cap_items_[fr], std::max(n, 1));
+        if (e == hipSuccess && n) e = hipMemcpyAsync(d_items_[fr], items[fr].data(), n * sizeof(WorkItem),
+                                                      hipMemcpyHostToDevice, st_);
+        if (e != hipSuccess) return hip_fail(e, "engine build items");
+    }
+    // CFL items: row ranges of about 8192 cells
+    std::vector<CflItem> cfl;
+    for (int s : order) {
+        const SoaGrid &g = boxes_[s].g;
+        const int rows = g.ny * g.nz, chunk = std::max(1, 8192 / g.nx);
+        for (int r = 0; r < rows; r += chunk) cfl.push_back({s, r, std::min(chunk, rows - r)});
+    }
+    ncfl_ = (int)cfl.size();
+    e = ensure_cap(&d_cfl_, &cap_cfl_, std::max(ncfl_, 1));
+    if (e == hipSuccess && ncfl_) e = hipMemcpyAsync(d_cfl_, cfl.data(), ncfl_ * sizeof(CflItem), hipMemcpyHostToDevice, st_);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_part_, part.data(), part.size() * 4, hipMemcpyHostToDevice, st_);
+    if (e == hipSuccess && !active_.empty())
+        e = hipMemcpyAsync(d_active_, active_.data(), active_.size() * 4, hipMemcpyHostToDevice, st_);
+    if (e != hipSuccess) return hip_fail(e, "engine build upload");
+    // persistent grid: blocks per CU from the occupancy of the heaviest variant
+    int occ = 1 << 30;
+    for (int fr = 0; fr < 3; fr++)
+        for (int adv = 0; adv < 2; adv++) {
+            int nb = 0;
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, sweep_kernel(fr, kmax_, adv), 64 * kBW,
+                                                             adv ? lds_advect_ : lds_fluidx_);
+            if (e != hipSuccess) return hip_fail(e, "occupancy");
+            occ = std::min(occ, std::max(nb, 1));
+        }
+    const int bpc = bpc_ > 0 ? std::min(bpc_, occ) : occ;
+    grid_ = std::max(1, bpc * ncu_);
+    e = hipStreamSynchronize(st_);
+    if (e != hipSuccess) return hip_fail(e, "engine build");
+    dirty_ = false;
+    return MHD_OK;
+}
+
+int Engine::enqueue_step()
+{
+    if (avg_)
+        hipLaunchKernelGGL((batch_cfl_kernel<true>), dim3(ncfl_), dim3(256), 0, st_, d_boxes_, d_cfl_, d_cmax_);
+    else
+        hipLaunchKernelGGL((batch_cfl_kernel<false>), dim3(ncfl_), dim3(256), 0, st_, d_boxes_, d_cfl_, d_cmax_);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(batch_dt_kernel, dim3(1), dim3(256), 0, st_, d_active_, (int)active_.size(), d_cmax_, d_dt_,
+                           d_t_, d_t64_, d_tf_, d_dts_, cap_, d_iter_, d_ctr_, 12 * kQueues);
+        e = hipGetLastError();
+    }
+    const int nq = policy_ == 0 ? kQueues : 1;
+    for (int s = 0; s < 12 && e == hipSuccess; s++) {
+        const int fr = kSweepFrameB[s];
+        const bool adv = kSweepOpB[s] == 1;
+        const size_t lds = adv ? lds_advect_ : lds_fluidx_;
+        if (policy_ == 2)
+            e = launch_grid(fr, kmax_, adv, nitems_[fr], lds, st_, d_boxes_, d_items_[fr], d_dt_, flip_before(s));
+        else
+            e = launch_sweep(fr, kmax_, adv, grid_, lds, st_, d_boxes_, d_items_[fr], d_part_ + fr * (kQueues + 1),
+                             d_ctr_ + s * kQueues, d_dt_, flip_before(s), nq);
+    }
+    return e == hipSuccess ? MHD_OK : hip_fail(e, "engine step");
+}
+
+void Engine::drop_graph()
+{
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    exec_ = nullptr;
+    graph_ = nullptr;
+}
+
+int Engine::capture()
+{
+    hipError_t e = hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+    const int rc = enqueue_step();
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(st_, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    graph_ = g;
+    exec_ = x;
+    return MHD_OK;
+}
+
+int Engine::step(int nsteps)
+{
+    if (nsteps < 0) return fail(MHD_ERR_ARG, "step: nsteps < 0");
+    if (active_.empty() || nsteps == 0) return MHD_OK;
+    if (dirty_) {
+        int rc = build();
+        if (rc) return rc;
+    }
+    if (!exec_) {
+        int rc = capture();
+        if (rc) return rc;
+    }
+    for (int i = 0; i < nsteps; i++) {
+        hipError_t e = hipGraphLaunch(exec_, st_);
+        if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    }
+    return MHD_OK;
+}
+
+int Engine::step_timed(int nsteps, float *ms)
+{
+    if (!ms) return fail(MHD_ERR_ARG, "step_timed: null ms");
+    if (dirty_) {
+        int rc = build();
+        if (rc) return rc;
+    }
+    if (!exec_ && !active_.empty()) {
+        int rc = capture();
+        if (rc) return rc;
+    }
+    hipError_t e = hipEventRecord(ev0_, st_);
+    int rc = e == hipSuccess ? step(nsteps) : hip_fail(e, "hipEventRecord");
+    if (rc) return rc;
+    e = hipEventRecord(ev1_, st_);
+    if (e == hipSuccess) e = hipEventSynchronize(ev1_);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, ev0_, ev1_);
+    return e == hipSuccess ? MHD_OK : hip_fail(e, "step_timed");
+}
+
+int Engine::sync()
+{
+    hipError_t e = hipStreamSynchronize(st_);
+    return e == hipSuccess ? MHD_OK : hip_fail(e, "engine sync");
+}
+
+// device copies of every active box's state and of the per-slot clocks
+int Engine::save_all(std::vector<float *> &saved)
+{
+    hipError_t e = hipStreamSynchronize(st_);
+    const size_t B = max_boxes_;
+    for (int s : active_) {
+        const size_t n = 8 * (size_t)boxes_[s].g.N;
+        float *p = nullptr;
+        if (e == hipSuccess) e = hipMalloc((void **)&p, n * 4);
+        if (e == hipSuccess) {
+            saved.push_back(p);
+            e = hipMemcpyAsync(p, boxes_[s].u, 5 * (size_t)boxes_[s].g.N * 4, hipMemcpyDeviceToDevice, st_);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(p + 5 * (size_t)boxes_[s].g.N, boxes_[s].b, 3 * (size_t)boxes_[s].g.N * 4,
+                                   hipMemcpyDeviceToDevice, st_);
+        }
+    }
+    float *clk = nullptr;
+    if (e == hipSuccess) e = hipMalloc((void **)&clk, B * (4 + 4 + 8 + 4));
+    if (e == hipSuccess) {
+        saved.push_back(clk);
+        char *c = (char *)clk;
+        e = hipMemcpyAsync(c, d_t_, B * 4, hipMemcpyDeviceToDevice, st_);
+        if (e == hipSuccess) e = hipMemcpyAsync(c + B * 4, d_dt_, B * 4, hipMemcpyDeviceToDevice, st_);
+        if (e == hipSuccess) e = hipMemcpyAsync(c + B * 8, d_t64_, B * 8, hipMemcpyDeviceToDevice, st_);
+        if (e == hipSuccess) e = hipMemcpyAsync(c + B * 16, d_iter_, B * 4, hipMemcpyDeviceToDevice, st_);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st_);
+    return e == hipSuccess ? MHD_OK : hip_fail(e, "save state");
+}
+
+int Engine::restore_all(std::vector<float *> &saved)
+{
+    hipError_t e = hipStreamSynchronize(st_);
+    const size_t B = max_boxes_;
+    size_t i = 0;
+    for (int s : active_) {
+        if (i >= saved.size()) break;
+        float *p = saved[i++];
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(boxes_[s].u, p, 5 * (size_t)boxes_[s].g.N * 4, hipMemcpyDeviceToDevice, st_);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(boxes_[s].b, p + 5 * (size_t)boxes_[s].g.N, 3 * (size_t)boxes_[s].g.N * 4,
+                               hipMemcpyDeviceToDevice, st_);
+    }
+    if (i < saved.size() && e == hipSuccess) {
+        char *c = (char *)saved[i];
+        e = hipMemcpyAsync(d_t_, c, B * 4, hipMemcpyDeviceToDevice, st_);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_dt_, c + B * 4, B * 4, hipMemcpyDeviceToDevice, st_);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_t64_, c + B * 8, B * 8, hipMemcpyDeviceToDevice, st_);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_iter_, c + B * 16, B * 4, hipMemcpyDeviceToDevice, st_);
+    }
+    if (e == hipSuccess) e = hipMemsetAsync(d_cmax_, 0, B * 8, st_);
+    if (e == hipSuccess) e = hipStreamSynchronize(st_);
+    for (float *p : saved) (void)hipFree(p);
+    saved.clear();
+    return e == hipSuccess ? MHD_OK : hip_fail(e, "restore state");
+}
+
+// Times candidate schedules on the batch itself (trial_steps each, after one
+// warm-up step), then restores every box's state and clock and keeps the
+// fastest.  Candidates: the persistent sweeps with {1, 2, 3, occupancy}
+// blocks per CU under both queue policies, and one block per work item.
+int Engine::autotune(int trial_steps, int *bpc_out, int *policy_out, float *best_ms)
+{
+    if (active_.empty()) return fail(MHD_ERR_STATE, "autotune: no active boxes");
+    if (trial_steps < 1) trial_steps = 1;
+    std::vector<float *> saved;
+    int rc = save_all(saved);
+    if (rc) return rc;
+    float best = INFINITY;
+    int bb = 0, bp = 0;
+    for (int policy = 0; policy < 3 && !rc; policy++)
+        for (int bpc : {1, 2, 3, 0}) {
+            if (policy == 2 && bpc != 0) continue;
+            rc = set_schedule(bpc, policy);
+            if (!rc) rc = step(1);
+            float ms = 0;
+            if (!rc) rc = step_timed(trial_steps, &ms);
+            if (rc) break;
+            if (ms < best) {
+                best = ms;
+                bb = bpc;
+                bp = policy;
+            }
+        }
+    const int rc2 = restore_all(saved);
+    if (rc) return rc;
+    if (rc2) return rc2;
+    rc = set_schedule(bb, bp);
+    if (bpc_out) *bpc_out = bb;
+    if (policy_out) *policy_out = bp;
+    if (best_ms) *best_ms = best / trial_steps;
+    return rc;
+}
+
+}  // namespace mhd
+
+// ===========================================================================
+// C ABI: mhd_batch_*
+// ===========================================================================
+using namespace mhd;
+
+struct mhd_batch {
+    Engine *eng;
+    std::vector<int> slot;
+};
+
+extern "C" {
+
+int mhd_batch_create(mhd_batch **out, int nbox, const int *dims, int avg_b)
+{
+    if (!out || !dims || nbox < 1) return fail(MHD_ERR_ARG, "mhd_batch_create: bad argument");
+    *out = nullptr;
+    for (int i = 0; i < nbox; i++) {
+        int rc = check_shape3(dims[3 * i], dims[3 * i + 1], dims[3 * i + 2]);
+        if (rc) return rc;
+    }
+    mhd_batch *bt = new mhd_batch();
+    bt->eng = new Engine(nbox, 1 << 14, avg_b);
+    int rc = bt->eng->init();
+    for (int i = 0; i < nbox && !rc; i++) {
+        int s = -1;
+        rc = bt->eng->add_box(dims[3 * i], dims[3 * i + 1], dims[3 * i + 2], &s);
+        bt->slot.push_back(s);
+    }
+    if (!rc) rc = bt->eng->set_active(bt->slot);
+    if (rc) {
+        delete bt->eng;
+        delete bt;
+        return rc;
+    }
+    *out = bt;
+    return MHD_OK;
+}
+
+int mhd_batch_destroy(mhd_batch *bt)
+{
+    if (!bt) return MHD_OK;
+    delete bt->eng;
+    delete bt;
+    return MHD_OK;
+}
+
+static int bslot(mhd_batch *bt, int box)
+{
+    if (!bt || box < 0 || box >= (int)bt->slot.size()) return -1;
+    return bt->slot[box];
+}
+
+int mhd_batch_set_state(mhd_batch *bt, int box, const float *h_u, const float *h_b)
+{
+    const int s = bslot(bt, box);
+    if (s < 0) return fail(MHD_ERR_ARG, "mhd_batch_set_state: bad box");
+    return bt->eng->set_state(s, h_u, h_b);
+}
+
+int mhd_batch_get_state(mhd_batch *bt, int box, float *h_u, float *h_b)
+{
+    const int s = bslot(bt, box);
+    if (s < 0) return fail(MHD_ERR_ARG, "mhd_batch_get_state: bad box");
+    return bt->eng->get_state(s, h_u, h_b);
+}
+
+int mhd_batch_step(mhd_batch *bt, int nsteps)
+{
+    if (!bt) return fail(MHD_ERR_ARG, "mhd_batch_step: null batch");
+    return bt->eng->step(nsteps);
+}
+
+int mhd_batch_step_timed(mhd_batch *bt, int nsteps, float *ms)
+{
+    if (!bt) return fail(MHD_ERR_ARG, "mhd_batch_step_timed: null batch");
+    return bt->eng->step_timed(nsteps, ms);
+}
+
+int mhd_batch_sync(mhd_batch *bt)
+{
+    if (!bt) return fail(MHD_ERR_ARG, "mhd_batch_sync: null batch");
+    return bt->eng->sync();
+}
+
+int mhd_batch_dts(mhd_batch *bt, int box, double *h_dts, int max, int *count)
+{
+    const int s = bslot(bt, box);
+    if (s < 0) return fail(MHD_ERR_ARG, "mhd_batch_dts: bad box");
+    return bt->eng->dts(s, h_dts, max, count);
+}
+
+int mhd_batch_set_schedule(mhd_batch *bt, int blocks_per_cu, int policy)
+{
+    if (!bt) return fail(MHD_ERR_ARG, "mhd_batch_set_schedule: null batch");
+    return bt->eng->set_schedule(blocks_per_cu, policy);
+}
+
+int mhd_batch_autotune(mhd_batch *bt, int trial_steps, int *blocks_per_cu, int *policy, float *ms_per_step)
+{
+    if (!bt) return fail(MHD_ERR_ARG, "mhd_batch_autotune: null batch");
+    return bt->eng->autotune(trial_steps, blocks_per_cu, policy, ms_per_step);
+}
+
+}  // extern "C"

@@ -6,9 +6,11 @@
 
 namespace mhd {
 
-// floor(f / d) as (f * m) >> 32, exact for 0 <= f < 2^32 / d
+// floor(f / d) as the high word of f * m, m = ceil(2^32 / d), exact for
+// 0 <= f < 2^32 / d; d == 1 is flagged (its m does not fit 32 bits)
 struct FastDiv {
-    unsigned long long m;
+    unsigned m;
+    int one;
 };
 FastDiv make_fastdiv(int d);
 

@@ -9,255 +9,9 @@
 // Every thread issues its loads for a batch of items before it writes any of
 // them to LDS, so the loads of a tile are in flight together.  The per-pencil
 // arithmetic is the shared core (mhd_core.h), unchanged.
-#include "mhd_core.h"
-#include "mhd_soa.h"
+#include "mhd_soa_tiles.h"
 
 namespace mhd {
-
-template <typename T> struct Vec16;
-template <> struct Vec16<float> {
-    using type = float4;
-    static constexpr int n = 4;
-};
-template <> struct Vec16<double> {
-    using type = double2;
-    static constexpr int n = 2;
-};
-template <typename T> using V16 = typename Vec16<T>::type;
-
-template <typename T> __device__ __forceinline__ T vget(const V16<T> &v, int e)
-{
-    if constexpr (sizeof(T) == 4)
-        return e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w;
-    else
-        return e == 0 ? v.x : v.y;
-}
-
-__device__ __forceinline__ int fdiv(int f, FastDiv d) { return (int)(((unsigned long long)(unsigned)f * d.m) >> 32); }
-
-__device__ __forceinline__ int wrapn(int j, int nj)
-{
-    while (j >= nj) j -= nj;
-    return j;
-}
-
-// ---------------------------------------------------------------------------
-// tile geometry
-// ---------------------------------------------------------------------------
-struct TileCtx {
-    int n, ns, j0, k, kp, km;  // pencil length, valid slots, first pencil, planes
-    long kb, kpb, kmb;         // cell offsets of planes k, k+1, k-1
-    bool vec;                  // the 16-byte vector path applies
-};
-
-template <int W> __device__ __forceinline__ TileCtx tile_ctx(const SoaBox &bx, int tile, int ntj)
-{
-    TileCtx t;
-    const Frame &f = bx.f;
-    t.n = f.ni;
-    t.k = tile / ntj;
-    t.j0 = (tile - t.k * ntj) * W;
-    t.ns = min(W, f.nj - t.j0);
-    t.kp = t.k + 1 == f.nk ? 0 : t.k + 1;
-    t.km = t.k == 0 ? f.nk - 1 : t.k - 1;
-    t.kb = (long)t.k * f.sk;
-    t.kpb = (long)t.kp * f.sk;
-    t.kmb = (long)t.km * f.sk;
-    // x frame: pencils of n values, n a multiple of the vector width; y/z
-    // frames: a full tile row of W cells starting at a vector boundary
-    t.vec = bx.vec_ok && (f.si == 1 || t.ns == W);
-    return t;
-}
-
-// ---------------------------------------------------------------------------
-// fluidx (Heterogeneous_MHD/C/fluidx.c:19-102 + tvd1.c + mhdflux.c)
-// LDS: su[W][5][n], sb[W][3][n], sb2n[n] (b2 of the pencil after the tile),
-//      sbk[W][n] (b3 of the tile's pencils at k+1)
-// ---------------------------------------------------------------------------
-template <int W> __host__ __device__ constexpr int fluidx_soa_words(int n) { return (9 * W + 1) * n; }
-
-template <typename T, int K, int W, int FR>
-__device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__restrict__ b, const SoaBox &bx,
-                                                const T *__restrict__ dtp, int tile, int ntj, T *smem)
-{
-    constexpr int VE = Vec16<T>::n;
-    static_assert(W == VE, "tile width is one 16-byte vector");
-    const Frame &f = bx.f;
-    const TileCtx t = tile_ctx<W>(bx, tile, ntj);
-    const int n = t.n, ns = t.ns, L = n / K;
-    const long N = bx.N;
-    T *su = smem, *sb = su + 5 * W * n, *sb2n = sb + 3 * W * n, *sbk = sb2n + n;
-    const int jn = wrapn(t.j0 + ns, f.nj);  // pencil after the tile (jp of the last one)
-    const int tid = threadIdx.x, nt = blockDim.x;
-
-    if (t.vec && f.si == 1) {
-        // x frame: items (segment, slot, component, vector index)
-        const int n4 = n / VE;
-        const int eU = ns * 5 * n4, eB = eU + ns * 3 * n4, eN = eB + n4, eK = eN + ns * n4;
-        for (int f0 = tid; f0 < eK; f0 += 4 * nt) {
-            V16<T> v[4];
-            T *dst[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int e = f0 + q * nt;
-                dst[q] = nullptr;
-                if (e < eK) {
-                    const T *src;
-                    if (e < eU) {
-                        const int sc = fdiv(e, bx.n4_div), i4 = e - sc * n4, s = sc / 5, c = sc - 5 * s;
-                        src = u + cu<FR>(c) * N + (long)(t.j0 + s) * f.sj + t.kb + i4 * VE;
-                        dst[q] = su + (s * 5 + c) * n + i4 * VE;
-                    } else if (e < eB) {
-                        const int r = e - eU, sc = fdiv(r, bx.n4_div), i4 = r - sc * n4, s = sc / 3, c = sc - 3 * s;
-                        src = b + cb<FR>(c) * N + (long)(t.j0 + s) * f.sj + t.kb + i4 * VE;
-                        dst[q] = sb + (s * 3 + c) * n + i4 * VE;
-                    } else if (e < eN) {
-                        const int i4 = e - eB;
-                        src = b + Comp<FR>::b[1] * N + (long)jn * f.sj + t.kb + i4 * VE;
-                        dst[q] = sb2n + i4 * VE;
-                    } else {
-                        const int r = e - eN, s = fdiv(r, bx.n4_div), i4 = r - s * n4;
-                        src = b + Comp<FR>::b[2] * N + (long)(t.j0 + s) * f.sj + t.kpb + i4 * VE;
-                        dst[q] = sbk + s * n + i4 * VE;
-                    }
-                    v[q] = *reinterpret_cast<const V16<T> *>(src);
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-                if (dst[q]) *reinterpret_cast<V16<T> *>(dst[q]) = v[q];
-        }
-    } else if (t.vec) {
-        // y/z frames, full tile: items (segment, component, row i); a row of
-        // the tile is one vector spanning the W slots
-        const int eU = 5 * n, eB = eU + 3 * n, eK = eB + n;
-        for (int f0 = tid; f0 < eK; f0 += 4 * nt) {
-            V16<T> v[4];
-            int dstride[4];
-            T *dbase[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int e = f0 + q * nt;
-                dbase[q] = nullptr;
-                if (e < eK) {
-                    const T *src;
-                    if (e < eU) {
-                        const int c = fdiv(e, bx.n_div), i = e - c * n;
-                        src = u + cu<FR>(c) * N + t.kb + (long)i * f.si + t.j0;
-                        dbase[q] = su + c * n + i;
-                        dstride[q] = 5 * n;
-                    } else if (e < eB) {
-                        const int r = e - eU, c = fdiv(r, bx.n_div), i = r - c * n;
-                        src = b + cb<FR>(c) * N + t.kb + (long)i * f.si + t.j0;
-                        dbase[q] = sb + c * n + i;
-                        dstride[q] = 3 * n;
-                    } else {
-                        const int i = e - eB;
-                        src = b + Comp<FR>::b[2] * N + t.kpb + (long)i * f.si + t.j0;
-                        dbase[q] = sbk + i;
-                        dstride[q] = n;
-                    }
-                    v[q] = *reinterpret_cast<const V16<T> *>(src);
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-                if (dbase[q])
-#pragma unroll
-                    for (int s = 0; s < W; s++) dbase[q][s * dstride[q]] = vget<T>(v[q], s);
-        }
-        // b2 of the pencil after the tile: one value per row
-        for (int i = tid; i < n; i += nt) sb2n[i] = b[Comp<FR>::b[1] * N + t.kb + (long)i * f.si + jn];
-    } else {
-        // general tile (partial, or vectors not aligned): one value per item
-        const int eU = ns * 5 * n, eB = eU + ns * 3 * n, eN = eB + n, eK = eN + ns * n;
-        for (int e = tid; e < eK; e += nt) {
-            int s, c, i;
-            if (e < eU) {
-                const int r = fdiv(e, bx.n_div);
-                i = e - r * n;
-                s = r / 5;
-                c = r - 5 * s;
-                su[(s * 5 + c) * n + i] =
-                    u[cu<FR>(c) * N + t.kb + (long)i * f.si + (long)(t.j0 + s) * f.sj];
-            } else if (e < eB) {
-                const int q = e - eU, r = fdiv(q, bx.n_div);
-                i = q - r * n;
-                s = r / 3;
-                c = r - 3 * s;
-                sb[(s * 3 + c) * n + i] =
-                    b[cb<FR>(c) * N + t.kb + (long)i * f.si + (long)(t.j0 + s) * f.sj];
-            } else if (e < eN) {
-                i = e - eB;
-                sb2n[i] = b[Comp<FR>::b[1] * N + t.kb + (long)i * f.si + (long)jn * f.sj];
-            } else {
-                const int q = e - eN;
-                s = fdiv(q, bx.n_div);
-                i = q - s * n;
-                sbk[s * n + i] = b[Comp<FR>::b[2] * N + t.kpb + (long)i * f.si + (long)(t.j0 + s) * f.sj];
-            }
-        }
-    }
-    __syncthreads();
-
-    const int w = threadIdx.x >> 6;
-    if (w < ns) {
-        const int l = lane_id();
-        const int lc = l % L;  // lanes >= L shadow lane l % L and store nothing
-        const int src_next = lc + 1 == L ? 0 : lc + 1, src_prev = lc == 0 ? L - 1 : lc - 1;
-        const T dt = *dtp;
-        T uu[K][5], bxc[K][3];
-#pragma unroll
-        for (int m = 0; m < 5; m++)
-#pragma unroll
-            for (int q = 0; q < K; q++) uu[q][m] = su[(w * 5 + m) * n + lc * K + q];
-#pragma unroll
-        for (int q = 0; q < K; q++) {
-            const int i = lc * K + q, ip = i + 1 == n ? 0 : i + 1;
-            const T b1 = sb[(w * 3 + 0) * n + i], b1p = sb[(w * 3 + 0) * n + ip];
-            const T b2 = sb[(w * 3 + 1) * n + i];
-            const T b2p = w + 1 < ns ? sb[((w + 1) * 3 + 1) * n + i] : sb2n[i];
-            const T b3 = sb[(w * 3 + 2) * n + i], b3p = sbk[w * n + i];
-            // face B to cell centre (fluidx.c:71-81)
-            bxc[q][0] = (T)0.5 * b1 + (T)0.5 * b1p;
-            bxc[q][1] = (T)0.5 * b2 + b2p / (T)2;
-            bxc[q][2] = (T)0.5 * b3 + b3p / (T)2;
-        }
-        tvd1_pencil<T, K>(uu, bxc, dt, src_next, src_prev);
-        if (l < L) {
-#pragma unroll
-            for (int m = 0; m < 5; m++)
-#pragma unroll
-                for (int q = 0; q < K; q++) su[(w * 5 + m) * n + lc * K + q] = uu[q][m];
-        }
-    }
-    __syncthreads();
-
-    // store the new u of the tile's pencils
-    if (t.vec && f.si == 1) {
-        const int n4 = n / VE, eU = ns * 5 * n4;
-        for (int e = tid; e < eU; e += nt) {
-            const int sc = fdiv(e, bx.n4_div), i4 = e - sc * n4, s = sc / 5, c = sc - 5 * s;
-            *reinterpret_cast<V16<T> *>(u + cu<FR>(c) * N + (long)(t.j0 + s) * f.sj + t.kb + i4 * VE) =
-                *reinterpret_cast<const V16<T> *>(su + (s * 5 + c) * n + i4 * VE);
-        }
-    } else if (t.vec) {
-        for (int e = tid; e < 5 * n; e += nt) {
-            const int c = fdiv(e, bx.n_div), i = e - c * n;
-            V16<T> v;
-            if constexpr (sizeof(T) == 4)
-                v = make_float4(su[c * n + i], su[(5 + c) * n + i], su[(10 + c) * n + i], su[(15 + c) * n + i]);
-            else
-                v = make_double2(su[c * n + i], su[(5 + c) * n + i]);
-            *reinterpret_cast<V16<T> *>(u + cu<FR>(c) * N + t.kb + (long)i * f.si + t.j0) = v;
-        }
-    } else {
-        for (int e = tid; e < ns * 5 * n; e += nt) {
-            const int r = fdiv(e, bx.n_div), i = e - r * n, s = r / 5, c = r - 5 * s;
-            u[cu<FR>(c) * N + t.kb + (long)i * f.si + (long)(t.j0 + s) * f.sj] = su[(s * 5 + c) * n + i];
-        }
-    }
-}
 
 template <typename T, int K, int W, int FR>
 __global__ void __launch_bounds__(64 * W)
@@ -266,277 +20,6 @@ __global__ void __launch_bounds__(64 * W)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     fluidx_soa_tile<T, K, W, FR>(u, b, bx, dtp, xcd_tile(blockIdx.x, gridDim.x), ntj,
                                  reinterpret_cast<T *>(smem_raw));
-}
-
-// ---------------------------------------------------------------------------
-// advectbyzx (Heterogeneous_MHD/C/advectbyzx.c:14-23, A1/A2/B1/B2 fused).
-// Rows of the flux work area X (3 ns + 1 of them, n values each):
-//   row s          FA(s), s = 0..ns   face velocity between pencils j0+s-1, j0+s (plane k)
-//   row ns+1+s     FB0(s)             between planes k-1 and k of pencil j0+s
-//   row 2ns+1+s    FB1(s)             between planes k and k+1 of pencil j0+s
-// Each row holds the smoothed-before face velocity (vx), then the shifted flux F.
-// LDS: X[3W+1][n], sb[W][3][n], sb2n[n] (b2 of the pencil after the tile),
-//      sb3p[W][n] (b3 of the tile's pencils at k+1)
-// ---------------------------------------------------------------------------
-template <int W> __host__ __device__ constexpr int advect_soa_words(int n) { return (7 * W + 2) * n; }
-
-template <typename T> __device__ __forceinline__ T face_v(T ma, T mb, T ra, T rb) { return (ma + mb) / (ra + rb); }
-
-template <typename T, int K, int W, int FR>
-__device__ __forceinline__ void advect_soa_tile(const T *__restrict__ u, const T *__restrict__ bin,
-                                                T *__restrict__ bout, const SoaBox &bx, const T *__restrict__ dtp,
-                                                int tile, int ntj, T *smem)
-{
-    constexpr int VE = Vec16<T>::n;
-    static_assert(W == VE, "tile width is one 16-byte vector");
-    const Frame &f = bx.f;
-    const TileCtx t = tile_ctx<W>(bx, tile, ntj);
-    const int n = t.n, ns = t.ns, L = n / K;
-    const long N = bx.N;
-    T *X = smem, *sb = X + (3 * W + 1) * n, *sb2n = sb + 3 * W * n, *sb3p = sb2n + n;
-    const int jn = wrapn(t.j0 + ns, f.nj), jm = wrapn(t.j0 - 1 + f.nj, f.nj);
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const T *rho = u, *mom = u + Comp<FR>::u[1] * N;
-    const int nrow = 3 * ns + 1;
-
-    if (t.vec && f.si == 1) {
-        const int n4 = n / VE;
-        // face velocities: rows x vectors
-        for (int e = tid; e < nrow * n4; e += nt) {
-            const int r = fdiv(e, bx.n4_div), i4 = e - r * n4;
-            long ca, cb;  // cell offsets of the two pencils (minus side first)
-            if (r <= ns) {
-                ca = (long)(r == 0 ? jm : t.j0 + r - 1) * f.sj + t.kb;
-                cb = (long)(r == ns ? jn : t.j0 + r) * f.sj + t.kb;
-            } else if (r <= 2 * ns) {
-                const long js = (long)(t.j0 + r - ns - 1) * f.sj;
-                ca = js + t.kmb;
-                cb = js + t.kb;
-            } else {
-                const long js = (long)(t.j0 + r - 2 * ns - 1) * f.sj;
-                ca = js + t.kb;
-                cb = js + t.kpb;
-            }
-            const long o = i4 * VE;
-            const V16<T> ma = *reinterpret_cast<const V16<T> *>(mom + ca + o);
-            const V16<T> mb = *reinterpret_cast<const V16<T> *>(mom + cb + o);
-            const V16<T> ra = *reinterpret_cast<const V16<T> *>(rho + ca + o);
-            const V16<T> rb = *reinterpret_cast<const V16<T> *>(rho + cb + o);
-            V16<T> vx;
-            if constexpr (sizeof(T) == 4)
-                vx = make_float4(face_v(ma.x, mb.x, ra.x, rb.x), face_v(ma.y, mb.y, ra.y, rb.y),
-                                 face_v(ma.z, mb.z, ra.z, rb.z), face_v(ma.w, mb.w, ra.w, rb.w));
-            else
-                vx = make_double2(face_v(ma.x, mb.x, ra.x, rb.x), face_v(ma.y, mb.y, ra.y, rb.y));
-            *reinterpret_cast<V16<T> *>(X + r * n + o) = vx;
-        }
-        // b of the tile (3 comps), b2 after the tile, b3 at k+1
-        const int eB = ns * 3 * n4, eN = eB + n4, eK = eN + ns * n4;
-        for (int f0 = tid; f0 < eK; f0 += 4 * nt) {
-            V16<T> v[4];
-            T *dst[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int e = f0 + q * nt;
-                dst[q] = nullptr;
-                if (e < eK) {
-                    const T *src;
-                    if (e < eB) {
-                        const int sc = fdiv(e, bx.n4_div), i4 = e - sc * n4, s = sc / 3, c = sc - 3 * s;
-                        src = bin + cb<FR>(c) * N + (long)(t.j0 + s) * f.sj + t.kb + i4 * VE;
-                        dst[q] = sb + (s * 3 + c) * n + i4 * VE;
-                    } else if (e < eN) {
-                        const int i4 = e - eB;
-                        src = bin + Comp<FR>::b[1] * N + (long)jn * f.sj + t.kb + i4 * VE;
-                        dst[q] = sb2n + i4 * VE;
-                    } else {
-                        const int r = e - eN, s = fdiv(r, bx.n4_div), i4 = r - s * n4;
-                        src = bin + Comp<FR>::b[2] * N + (long)(t.j0 + s) * f.sj + t.kpb + i4 * VE;
-                        dst[q] = sb3p + s * n + i4 * VE;
-                    }
-                    v[q] = *reinterpret_cast<const V16<T> *>(src);
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-                if (dst[q]) *reinterpret_cast<V16<T> *>(dst[q]) = v[q];
-        }
-    } else if (t.vec) {
-        // y/z frames, full tile: one item per row i does every face velocity
-        for (int i = tid; i < n; i += nt) {
-            const long ro = (long)i * f.si;
-            const long ck = t.kb + ro + t.j0, ckm = t.kmb + ro + t.j0, ckp = t.kpb + ro + t.j0;
-            const V16<T> m0 = *reinterpret_cast<const V16<T> *>(mom + ck);
-            const V16<T> r0 = *reinterpret_cast<const V16<T> *>(rho + ck);
-            const V16<T> mm = *reinterpret_cast<const V16<T> *>(mom + ckm);
-            const V16<T> rm = *reinterpret_cast<const V16<T> *>(rho + ckm);
-            const V16<T> mp = *reinterpret_cast<const V16<T> *>(mom + ckp);
-            const V16<T> rp = *reinterpret_cast<const V16<T> *>(rho + ckp);
-            const T mlo = mom[t.kb + ro + jm], rlo = rho[t.kb + ro + jm];
-            const T mhi = mom[t.kb + ro + jn], rhi = rho[t.kb + ro + jn];
-            T mk[W + 2], rk[W + 2];
-            mk[0] = mlo;
-            rk[0] = rlo;
-#pragma unroll
-            for (int s = 0; s < W; s++) {
-                mk[s + 1] = vget<T>(m0, s);
-                rk[s + 1] = vget<T>(r0, s);
-            }
-            mk[W + 1] = mhi;
-            rk[W + 1] = rhi;
-#pragma unroll
-            for (int s = 0; s <= W; s++) X[s * n + i] = face_v(mk[s], mk[s + 1], rk[s], rk[s + 1]);
-#pragma unroll
-            for (int s = 0; s < W; s++) {
-                X[(W + 1 + s) * n + i] = face_v(vget<T>(mm, s), mk[s + 1], vget<T>(rm, s), rk[s + 1]);
-                X[(2 * W + 1 + s) * n + i] = face_v(mk[s + 1], vget<T>(mp, s), rk[s + 1], vget<T>(rp, s));
-            }
-        }
-        const int eB = 3 * n, eK = eB + n;
-        for (int f0 = tid; f0 < eK; f0 += 4 * nt) {
-            V16<T> v[4];
-            T *dbase[4];
-            int dstride[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int e = f0 + q * nt;
-                dbase[q] = nullptr;
-                if (e < eK) {
-                    const T *src;
-                    if (e < eB) {
-                        const int c = fdiv(e, bx.n_div), i = e - c * n;
-                        src = bin + cb<FR>(c) * N + t.kb + (long)i * f.si + t.j0;
-                        dbase[q] = sb + c * n + i;
-                        dstride[q] = 3 * n;
-                    } else {
-                        const int i = e - eB;
-                        src = bin + Comp<FR>::b[2] * N + t.kpb + (long)i * f.si + t.j0;
-                        dbase[q] = sb3p + i;
-                        dstride[q] = n;
-                    }
-                    v[q] = *reinterpret_cast<const V16<T> *>(src);
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-                if (dbase[q])
-#pragma unroll
-                    for (int s = 0; s < W; s++) dbase[q][s * dstride[q]] = vget<T>(v[q], s);
-        }
-        for (int i = tid; i < n; i += nt) sb2n[i] = bin[Comp<FR>::b[1] * N + t.kb + (long)i * f.si + jn];
-    } else {
-        // general tile: one value per item
-        for (int e = tid; e < nrow * n; e += nt) {
-            const int r = fdiv(e, bx.n_div), i = e - r * n;
-            const long ro = (long)i * f.si;
-            long ca, cb;
-            if (r <= ns) {
-                ca = (long)(r == 0 ? jm : t.j0 + r - 1) * f.sj + t.kb + ro;
-                cb = (long)(r == ns ? jn : t.j0 + r) * f.sj + t.kb + ro;
-            } else if (r <= 2 * ns) {
-                const long js = (long)(t.j0 + r - ns - 1) * f.sj + ro;
-                ca = js + t.kmb;
-                cb = js + t.kb;
-            } else {
-                const long js = (long)(t.j0 + r - 2 * ns - 1) * f.sj + ro;
-                ca = js + t.kb;
-                cb = js + t.kpb;
-            }
-            X[r * n + i] = face_v(mom[ca], mom[cb], rho[ca], rho[cb]);
-        }
-        const int eB = ns * 3 * n, eN = eB + n, eK = eN + ns * n;
-        for (int e = tid; e < eK; e += nt) {
-            if (e < eB) {
-                const int r = fdiv(e, bx.n_div), i = e - r * n, s = r / 3, c = r - 3 * s;
-                sb[(s * 3 + c) * n + i] = bin[cb<FR>(c) * N + t.kb + (long)i * f.si + (long)(t.j0 + s) * f.sj];
-            } else if (e < eN) {
-                const int i = e - eB;
-                sb2n[i] = bin[Comp<FR>::b[1] * N + t.kb + (long)i * f.si + (long)jn * f.sj];
-            } else {
-                const int q = e - eN, s = fdiv(q, bx.n_div), i = q - s * n;
-                sb3p[s * n + i] = bin[Comp<FR>::b[2] * N + t.kpb + (long)i * f.si + (long)(t.j0 + s) * f.sj];
-            }
-        }
-    }
-    __syncthreads();
-
-    const int w = threadIdx.x >> 6;
-    const int l = lane_id();
-    const int lc = l % L;
-    const int src_next = lc + 1 == L ? 0 : lc + 1, src_prev = lc == 0 ? L - 1 : lc - 1;
-    const T dt = *dtp;
-    // (the strided vector path numbers rows with W, equal to ns there)
-    for (int it = w; it < nrow; it += W) {
-        T vx[K], vxm[K], vxn[K], vs[K], bi[K], bnew[K], F[K];
-        T *row = X + it * n;
-#pragma unroll
-        for (int q = 0; q < K; q++) vx[q] = row[lc * K + q];
-        shift_prev<T, K>(vx, vxm, src_prev);
-        shift_next<T, K>(vx, vxn, src_next);
-#pragma unroll
-        for (int q = 0; q < K; q++) {
-            // A1.c:41 (vx[i-1] + vx[i+1] + 2.0*vx)/4.0
-            vs[q] = (T)(((double)(vxm[q] + vxn[q]) + 2.0 * (double)vx[q]) / 4.0);
-            const int i = lc * K + q;
-            if (it <= ns)
-                bi[q] = it < ns ? sb[(it * 3 + 1) * n + i] : sb2n[i];
-            else if (it <= 2 * ns)
-                bi[q] = sb[((it - ns - 1) * 3 + 2) * n + i];
-            else
-                bi[q] = sb3p[(it - 2 * ns - 1) * n + i];
-        }
-        tvdb_pencil<T, K>(bi, vs, dt, bnew, F, src_next, src_prev);
-        if (l < L) {
-#pragma unroll
-            for (int q = 0; q < K; q++) {
-                const int i = lc * K + q;
-                row[i] = F[q];
-                // new b2 (FA) / b3 (FB0) of the tile's own pencils; only this
-                // item reads that field row, so it is overwritten in place
-                if (it < ns) sb[(it * 3 + 1) * n + i] = bnew[q];
-                if (it > ns && it <= 2 * ns) sb[((it - ns - 1) * 3 + 2) * n + i] = bnew[q];
-            }
-        }
-    }
-    __syncthreads();
-    // b1 = (((b1 - FA_j) + FA_{j+1}) - FB_k) + FB_{k+1}  (A1.c:55, A2.c:23, B1.c:54, B2.c:23)
-    if (w < ns && l < L) {
-#pragma unroll
-        for (int q = 0; q < K; q++) {
-            const int i = lc * K + q;
-            T b1 = sb[(w * 3) * n + i];
-            b1 = b1 - X[w * n + i];
-            b1 = b1 + X[(w + 1) * n + i];
-            b1 = b1 - X[(ns + 1 + w) * n + i];
-            b1 = b1 + X[(2 * ns + 1 + w) * n + i];
-            sb[(w * 3) * n + i] = b1;
-        }
-    }
-    __syncthreads();
-
-    if (t.vec && f.si == 1) {
-        const int n4 = n / VE, eB = ns * 3 * n4;
-        for (int e = tid; e < eB; e += nt) {
-            const int sc = fdiv(e, bx.n4_div), i4 = e - sc * n4, s = sc / 3, c = sc - 3 * s;
-            *reinterpret_cast<V16<T> *>(bout + cb<FR>(c) * N + (long)(t.j0 + s) * f.sj + t.kb + i4 * VE) =
-                *reinterpret_cast<const V16<T> *>(sb + (s * 3 + c) * n + i4 * VE);
-        }
-    } else if (t.vec) {
-        for (int e = tid; e < 3 * n; e += nt) {
-            const int c = fdiv(e, bx.n_div), i = e - c * n;
-            V16<T> v;
-            if constexpr (sizeof(T) == 4)
-                v = make_float4(sb[c * n + i], sb[(3 + c) * n + i], sb[(6 + c) * n + i], sb[(9 + c) * n + i]);
-            else
-                v = make_double2(sb[c * n + i], sb[(3 + c) * n + i]);
-            *reinterpret_cast<V16<T> *>(bout + cb<FR>(c) * N + t.kb + (long)i * f.si + t.j0) = v;
-        }
-    } else {
-        for (int e = tid; e < ns * 3 * n; e += nt) {
-            const int r = fdiv(e, bx.n_div), i = e - r * n, s = r / 3, c = r - 3 * s;
-            bout[cb<FR>(c) * N + t.kb + (long)i * f.si + (long)(t.j0 + s) * f.sj] = sb[(s * 3 + c) * n + i];
-        }
-    }
 }
 
 template <typename T, int K, int W, int FR>
@@ -661,7 +144,8 @@ __global__ void __launch_bounds__(256) aos_soa_kernel(T *__restrict__ soa, T *__
 FastDiv make_fastdiv(int d)
 {
     FastDiv r;
-    r.m = ((1ull << 32) + (unsigned long long)d - 1) / (unsigned long long)d;
+    r.one = d <= 1;
+    r.m = d <= 1 ? 0u : (unsigned)(((1ull << 32) + (unsigned long long)d - 1) / (unsigned long long)d);
     return r;
 }
 

@@ -66,6 +66,16 @@ SIGNATURES = {
     "mhd_ctx_layout": (_I, [_P, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long),
                             ctypes.POINTER(ctypes.c_long)]),
     "mhd_ctx_profile_step": (_I, [_P, _I, _FP]),
+    "mhd_batch_create": (_I, [ctypes.POINTER(_P), _I, _IP, _I]),
+    "mhd_batch_destroy": (_I, [_P]),
+    "mhd_batch_set_state": (_I, [_P, _I, _P, _P]),
+    "mhd_batch_get_state": (_I, [_P, _I, _P, _P]),
+    "mhd_batch_step": (_I, [_P, _I]),
+    "mhd_batch_step_timed": (_I, [_P, _I, _FP]),
+    "mhd_batch_sync": (_I, [_P]),
+    "mhd_batch_dts": (_I, [_P, _I, _DP, _I, _IP]),
+    "mhd_batch_set_schedule": (_I, [_P, _I, _I]),
+    "mhd_batch_autotune": (_I, [_P, _I, _IP, _IP, _FP]),
 }
 
 
@@ -270,6 +280,70 @@ class Simulation:
         if self.ctx:
             lib().mhd_ctx_destroy(self.ctx)
             self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many independent boxes advanced together (the batched form), FP32."""
+
+    def __init__(self, dims, avg_b=False):
+        self.dims = [tuple(int(x) for x in d) for d in dims]
+        flat = (ctypes.c_int * (3 * len(self.dims)))(*[x for d in self.dims for x in d])
+        p = ctypes.c_void_p()
+        check(lib().mhd_batch_create(ctypes.byref(p), len(self.dims), flat, int(avg_b)))
+        self.bt = p.value
+
+    def ncell(self, i):
+        nx, ny, nz = self.dims[i]
+        return nx * ny * nz
+
+    def set_state(self, i, u, b):
+        u = np.ascontiguousarray(u, np.float32)
+        b = np.ascontiguousarray(b, np.float32)
+        assert u.size == 5 * self.ncell(i) and b.size == 3 * self.ncell(i)
+        check(lib().mhd_batch_set_state(self.bt, i, u.ctypes.data, b.ctypes.data))
+
+    def get_state(self, i):
+        u = np.empty(5 * self.ncell(i), np.float32)
+        b = np.empty(3 * self.ncell(i), np.float32)
+        check(lib().mhd_batch_get_state(self.bt, i, u.ctypes.data, b.ctypes.data))
+        return u, b
+
+    def step(self, n=1):
+        check(lib().mhd_batch_step(self.bt, int(n)))
+
+    def step_timed(self, n):
+        ms = ctypes.c_float()
+        check(lib().mhd_batch_step_timed(self.bt, int(n), ctypes.byref(ms)))
+        return ms.value
+
+    def sync(self):
+        check(lib().mhd_batch_sync(self.bt))
+
+    def dts(self, i, maxn=1 << 14):
+        cnt = ctypes.c_int()
+        out = np.zeros(maxn, np.float64)
+        check(lib().mhd_batch_dts(self.bt, i, out.ctypes.data_as(_DP), maxn, ctypes.byref(cnt)))
+        return out[:min(cnt.value, maxn)].astype(np.float32)
+
+    def set_schedule(self, blocks_per_cu, policy):
+        check(lib().mhd_batch_set_schedule(self.bt, int(blocks_per_cu), int(policy)))
+
+    def autotune(self, trial_steps=3):
+        bpc, pol, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+        check(lib().mhd_batch_autotune(self.bt, int(trial_steps), ctypes.byref(bpc), ctypes.byref(pol),
+                                       ctypes.byref(ms)))
+        return {"blocks_per_cu": bpc.value, "policy": pol.value, "ms_per_step": ms.value}
+
+    def close(self):
+        if self.bt:
+            lib().mhd_batch_destroy(self.bt)
+            self.bt = None
 
     def __del__(self):
         try:
