@@ -110,6 +110,65 @@ def bench_batched(mhdgpu, steps=100, nbox=64, seed=3):
             "boxes_identical_to_plain": bool(same)}
 
 
+def served_workload(n_req=64, seed=5, rate=20.0):
+    """Poisson arrivals (mean `rate` req/s, seeded) of requests with N uniform
+    over {32,48,64,96,128,192} and steps uniform in [50,500]; each initial
+    state is the random-phase state of its size, cyclically shifted by
+    seeded offsets (a distinct periodic state per request)."""
+    from mhdgpu.ics import random_phase
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice(np.array([32, 48, 64, 96, 128, 192]), n_req)
+    steps = rng.integers(50, 501, n_req)
+    gaps = rng.exponential(1.0 / rate, n_req)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    base = {int(n): random_phase(int(n), seed=int(n)) for n in sorted(set(sizes.tolist()))}
+    states = []
+    for i, n in enumerate(sizes.tolist()):
+        sh = rng.integers(0, n, 3)
+        u, b = base[n]
+        u = np.roll(u.reshape(n, n, n, 5), tuple(sh), axis=(0, 1, 2)).ravel()
+        b = np.roll(b.reshape(n, n, n, 3), tuple(sh), axis=(0, 1, 2)).ravel()
+        states.append((u, b))
+    return sizes.tolist(), steps.tolist(), arrivals.tolist(), states
+
+
+def run_served(mhdgpu, mode, sizes, steps, arrivals, states):
+    sv = mhdgpu.Server(mode=mode)
+    t0 = time.perf_counter()
+    done = {}
+    for i in range(len(sizes)):
+        while time.perf_counter() - t0 < arrivals[i]:
+            for r in sv.poll():
+                done[r["id"]] = r
+            time.sleep(0.0005)
+        n = sizes[i]
+        sv.submit((n, n, n), steps[i], *states[i])
+    while len(done) < len(sizes):
+        for r in sv.poll():
+            done[r["id"]] = r
+        time.sleep(0.001)
+    st = sv.stats()
+    sv.close()
+    lat = np.array([r["latency_s"] for r in done.values()])
+    ok = all(r["rc"] == 0 for r in done.values())
+    return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean()), "zone_updates": st["zone_updates"], "span_s": st["span_s"],
+            "zone_updates_per_s": st["zone_updates"] / st["span_s"], "gpu_busy_s": st["gpu_busy_s"],
+            "rebuilds": int(st["rebuilds"]), "all_ok": ok}
+
+
+def bench_served(mhdgpu, n_req=64):
+    sizes, steps, arrivals, states = served_workload(n_req)
+    log(f"bench: served, {n_req} requests over {arrivals[-1]:.2f} s")
+    batched = run_served(mhdgpu, 0, sizes, steps, arrivals, states)
+    log("bench: served FIFO baseline")
+    fifo = run_served(mhdgpu, 1, sizes, steps, arrivals, states)
+    return {"requests": n_req, "arrival_rate_per_s": 20.0, "arrival_span_s": arrivals[-1],
+            "offered_zone_updates": float(sum(n ** 3 * k for n, k in zip(sizes, steps))),
+            "continuous_batching": batched, "fifo_one_at_a_time": fifo,
+            "throughput_ratio": batched["zone_updates_per_s"] / fifo["zone_updates_per_s"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +177,8 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--requests", type=int, default=64)
     ap.add_argument("--mode", choices=["plain", "copy"], default="plain",
                     help="copy: time the plain copy kernel on 1 GiB (counter calibration)")
     args = ap.parse_args()
@@ -200,6 +261,8 @@ def main():
     }
     if not args.no_batched:
         out["batched"] = bench_batched(mhdgpu)
+    if not args.no_served:
+        out["served"] = bench_served(mhdgpu, args.requests)
     if not args.no_cpu:
         log("bench: CPU baseline")
         out["cpu_baseline"] = cpu_baseline(u0, b0, n)

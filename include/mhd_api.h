@@ -179,6 +179,32 @@ int mhd_batch_set_schedule(mhd_batch *bt, int blocks_per_cu, int policy);
  * restores every box's state and clock, keeps the fastest */
 int mhd_batch_autotune(mhd_batch *bt, int trial_steps, int *blocks_per_cu, int *policy, float *ms_per_step);
 
+/* ---- served form: a request loop with continuous batching ---------------
+ * No counterpart in the reference.  mode 0: requests are admitted in arrival
+ * order while their device state fits hbm_budget bytes (0 = 90% of the free
+ * device memory), at most max_active run at once (0 = 256) and, beyond the
+ * first, their cells stay under max_active_cells (0 = 16 Mi, enough to keep
+ * every CU busy); admitted requests stay resident and advance together
+ * through the batched engine, new ones joining at the next step boundary
+ * (results are bit-identical to running each request alone).  mode 1: the FIFO baseline, one request at a time
+ * through the device-resident plain path.  A service thread does the GPU
+ * work; submit/poll/pending/stats are thread-safe.  The caller keeps h_u,h_b
+ * (input, AoS) and out_u,out_b (result) alive until the request is polled.
+ * A request larger than the whole budget is refused with MHD_ERR_FULL. */
+typedef struct mhd_server mhd_server;
+int mhd_server_create(mhd_server **sv, int mode, size_t hbm_budget, int max_active, long long max_active_cells);
+int mhd_server_submit(mhd_server *sv, int nx, int ny, int nz, int nsteps, const float *h_u, const float *h_b,
+                      float *out_u, float *out_b, long long *req_id);
+/* completed requests since the last poll: id, time from submit to result,
+ * time spent queued before admission, return code; *n of at most max */
+int mhd_server_poll(mhd_server *sv, long long *ids, double *latency_s, double *queue_s, int *rcs, int max, int *n);
+int mhd_server_pending(mhd_server *sv, int *queued, int *active);
+/* out[7]: zone-updates, seconds first arrival -> last completion, GPU busy
+ * seconds, batched steps, work-list rebuilds, bytes resident, budget */
+int mhd_server_stats(mhd_server *sv, double *out7);
+/* finishes every submitted request, then stops the service thread */
+int mhd_server_destroy(mhd_server *sv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -86,7 +86,7 @@ private:
     CflItem *d_cfl_ = nullptr;
     int nitems_[3] = {0, 0, 0}, ncfl_ = 0, cap_items_[3] = {0, 0, 0}, cap_cfl_ = 0;
     int kmax_ = 1, lds_fluidx_ = 0, lds_advect_ = 0;
-    int bpc_ = 0, policy_ = 0, grid_ = 0, ncu_ = 0;
+    int bpc_ = 0, policy_ = 2, grid_ = 0, ncu_ = 0;  // default: one block per item (fastest measured)
     bool dirty_ = true;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;

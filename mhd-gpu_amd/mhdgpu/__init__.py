@@ -76,6 +76,12 @@ SIGNATURES = {
     "mhd_batch_dts": (_I, [_P, _I, _DP, _I, _IP]),
     "mhd_batch_set_schedule": (_I, [_P, _I, _I]),
     "mhd_batch_autotune": (_I, [_P, _I, _IP, _IP, _FP]),
+    "mhd_server_create": (_I, [ctypes.POINTER(_P), _I, _SZ, _I, ctypes.c_longlong]),
+    "mhd_server_submit": (_I, [_P, _I, _I, _I, _I, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_longlong)]),
+    "mhd_server_poll": (_I, [_P, ctypes.POINTER(ctypes.c_longlong), _DP, _DP, _IP, _I, _IP]),
+    "mhd_server_pending": (_I, [_P, _IP, _IP]),
+    "mhd_server_stats": (_I, [_P, _DP]),
+    "mhd_server_destroy": (_I, [_P]),
 }
 
 
@@ -344,6 +350,65 @@ class Batch:
         if self.bt:
             lib().mhd_batch_destroy(self.bt)
             self.bt = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """The request loop: mode 0 continuous batching, mode 1 FIFO one at a time."""
+
+    def __init__(self, mode=0, hbm_budget=0, max_active=256, max_active_cells=0):
+        p = ctypes.c_void_p()
+        check(lib().mhd_server_create(ctypes.byref(p), int(mode), int(hbm_budget), int(max_active),
+                                      int(max_active_cells)))
+        self.sv = p.value
+        self._keep = {}
+
+    def submit(self, dims, nsteps, u, b):
+        """Queue a request; returns (id, (out_u, out_b)) - the outputs are
+        filled when the request shows up in poll()."""
+        u = np.ascontiguousarray(u, np.float32)
+        b = np.ascontiguousarray(b, np.float32)
+        ou, ob = np.empty_like(u), np.empty_like(b)
+        rid = ctypes.c_longlong()
+        check(lib().mhd_server_submit(self.sv, dims[0], dims[1], dims[2], int(nsteps), u.ctypes.data, b.ctypes.data,
+                                      ou.ctypes.data, ob.ctypes.data, ctypes.byref(rid)))
+        self._keep[rid.value] = (u, b, ou, ob)
+        return rid.value, (ou, ob)
+
+    def poll(self, maxn=1024):
+        ids = (ctypes.c_longlong * maxn)()
+        lat = np.zeros(maxn)
+        qs = np.zeros(maxn)
+        rcs = (ctypes.c_int * maxn)()
+        n = ctypes.c_int()
+        check(lib().mhd_server_poll(self.sv, ids, lat.ctypes.data_as(_DP), qs.ctypes.data_as(_DP), rcs, maxn,
+                                    ctypes.byref(n)))
+        out = []
+        for i in range(n.value):
+            out.append({"id": ids[i], "latency_s": float(lat[i]), "queue_s": float(qs[i]), "rc": rcs[i]})
+            self._keep.pop(ids[i], None)
+        return out
+
+    def pending(self):
+        q, a = ctypes.c_int(), ctypes.c_int()
+        check(lib().mhd_server_pending(self.sv, ctypes.byref(q), ctypes.byref(a)))
+        return q.value, a.value
+
+    def stats(self):
+        out = np.zeros(7)
+        check(lib().mhd_server_stats(self.sv, out.ctypes.data_as(_DP)))
+        keys = ["zone_updates", "span_s", "gpu_busy_s", "steps", "rebuilds", "resident_bytes", "budget_bytes"]
+        return dict(zip(keys, out.tolist()))
+
+    def close(self):
+        if self.sv:
+            lib().mhd_server_destroy(self.sv)
+            self.sv = None
 
     def __del__(self):
         try:
