@@ -7,35 +7,51 @@
 
 namespace mhd {
 
-template <typename T, int K>
-__device__ __forceinline__ void mhdflux_pencil(const T (&uu)[K][5], const T (&bx)[K][3], T (&v)[K][5], T &c)
+// FAST (FP32 only): the divisions of mhdflux run through div_by; the
+// returned flag says whether every operand was in its exact range.
+template <typename T, int K, bool FAST>
+__device__ __forceinline__ bool mhdflux_pencil(const T (&uu)[K][5], const T (&bx)[K][3], T (&v)[K][5], T &c)
 {
+    bool ok = true;
     T cl = (T)0;
 #pragma unroll
     for (int q = 0; q < K; q++) {
         T ci;
-        mhdflux_cell<T>(uu[q], bx[q], v[q], ci);
+        if constexpr (FAST)
+            mhdflux_cell_fast(uu[q], bx[q], v[q], ci, ok);
+        else
+            mhdflux_cell<T>(uu[q], bx[q], v[q], ci);
         cl = fmax(cl, ci);
     }
     c = wave_max(cl);
     if (c > (T)0) {
+        if constexpr (FAST) {
+            const Recip rc = recip_of(c, ok);
 #pragma unroll
-        for (int q = 0; q < K; q++)
+            for (int q = 0; q < K; q++)
 #pragma unroll
-            for (int m = 0; m < 5; m++) v[q][m] = v[q][m] / c;
+                for (int m = 0; m < 5; m++) v[q][m] = div_by(v[q][m], rc, ok);
+        } else {
+#pragma unroll
+            for (int q = 0; q < K; q++)
+#pragma unroll
+                for (int m = 0; m < 5; m++) v[q][m] = v[q][m] / c;
+        }
     }
+    return ok;
 }
 
 // tvd1 on a pencil held K cells per lane; uu is updated in place.
-template <typename T, int K>
-__device__ __forceinline__ void tvd1_pencil(T (&uu)[K][5], const T (&bx)[K][3], T dt, int src_next, int src_prev)
+template <typename T, int K, bool FAST = false>
+__device__ __forceinline__ bool tvd1_pencil(T (&uu)[K][5], const T (&bx)[K][3], T dt, int src_next, int src_prev)
 {
+    bool ok = true;
     const T h = (T)(-1.0 * (double)dt / 2.0);
     const T mdt = (T)(-1.0 * (double)dt);
     T v[K][5], c;
     T u1[K][5];
     // first half step (tvd1.c:34-49)
-    mhdflux_pencil<T, K>(uu, bx, v, c);
+    ok = mhdflux_pencil<T, K, FAST>(uu, bx, v, c) && ok;
 #pragma unroll
     for (int m = 0; m < 5; m++) {
         T wl[K], fr[K], wln[K], fl[K], fx[K], fxp[K];
@@ -56,7 +72,7 @@ __device__ __forceinline__ void tvd1_pencil(T (&uu)[K][5], const T (&bx)[K][3], 
         for (int q = 0; q < K; q++) u1[q][m] = uu[q][m] + h * (fx[q] - fxp[q]);
     }
     // full step with limited fluxes (tvd1.c:52-116)
-    mhdflux_pencil<T, K>(u1, bx, v, c);
+    ok = mhdflux_pencil<T, K, FAST>(u1, bx, v, c) && ok;
 #pragma unroll
     for (int m = 0; m < 5; m++) {
         T wl[K], fr[K], frn[K], frp[K], wln[K], fl[K], fln[K], flp[K], fx[K], fxp[K];
@@ -87,6 +103,7 @@ __device__ __forceinline__ void tvd1_pencil(T (&uu)[K][5], const T (&bx)[K][3], 
 #pragma unroll
         for (int q = 0; q < K; q++) uu[q][m] = uu[q][m] + mdt * (fx[q] - fxp[q]);
     }
+    return ok;
 }
 
 // tvdb on a pencil held K cells per lane (tvdb.c:10-91): bin -> bnew, and

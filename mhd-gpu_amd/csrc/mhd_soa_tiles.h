@@ -69,7 +69,7 @@ template <int W> __device__ __forceinline__ TileCtx tile_ctx(const SoaBox &bx, i
 // ---------------------------------------------------------------------------
 template <int W> __host__ __device__ constexpr int fluidx_soa_words(int n) { return (9 * W + 1) * n; }
 
-template <typename T, int K, int W, int FR>
+template <typename T, int K, int W, int FR, bool FASTDIV = false>
 __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__restrict__ b, const SoaBox &bx,
                                                 const T *__restrict__ dtp, int tile, int ntj, T *smem)
 {
@@ -216,7 +216,24 @@ __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__re
             bxc[q][1] = (T)0.5 * b2 + b2p / (T)2;
             bxc[q][2] = (T)0.5 * b3 + b3p / (T)2;
         }
-        tvd1_pencil<T, K>(uu, bxc, dt, src_next, src_prev);
+        bool exact = true;
+        if constexpr (sizeof(T) == 4 && FASTDIV) {
+            // fast divisions; if any lane met an operand outside their exact
+            // range, redo the pencil with true divisions (wave-uniform branch)
+            exact = __all(tvd1_pencil<T, K, true>(uu, bxc, dt, src_next, src_prev));
+            if (!exact) {
+                // reload from LDS (the compiler must not keep the inputs live
+                // across the fast attempt for this rare case)
+                asm volatile("" ::: "memory");
+#pragma unroll
+                for (int m = 0; m < 5; m++)
+#pragma unroll
+                    for (int q = 0; q < K; q++) uu[q][m] = su[(w * 5 + m) * n + lc * K + q];
+            }
+        } else {
+            exact = false;
+        }
+        if (!exact) tvd1_pencil<T, K, false>(uu, bxc, dt, src_next, src_prev);
         if (l < L) {
 #pragma unroll
             for (int m = 0; m < 5; m++)
