@@ -83,44 +83,43 @@ __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__re
     const int jn = wrapn(t.j0 + ns, f.nj);  // pencil after the tile (jp of the last one)
     const int tid = threadIdx.x, nt = blockDim.x;
 
-    if (t.vec && f.si == 1) {
-        // x frame: items (segment, slot, component, vector index)
+    if (FR == 0 && t.vec) {
+        // x frame: items (segment, slot, component, vector index); four items
+        // per thread are loaded before any is stored
         const int n4 = n / VE;
         const int eU = ns * 5 * n4, eB = eU + ns * 3 * n4, eN = eB + n4, eK = eN + ns * n4;
-        for (int f0 = tid; f0 < eK; f0 += 4 * nt) {
-            V16<T> v[4];
-            T *dst[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int e = f0 + q * nt;
-                dst[q] = nullptr;
-                if (e < eK) {
-                    const T *src;
-                    if (e < eU) {
-                        const int sc = fdiv(e, bx.n4_div), i4 = e - sc * n4, s = sc / 5, c = sc - 5 * s;
-                        src = u + cu<FR>(c) * N + (long)(t.j0 + s) * f.sj + t.kb + i4 * VE;
-                        dst[q] = su + (s * 5 + c) * n + i4 * VE;
-                    } else if (e < eB) {
-                        const int r = e - eU, sc = fdiv(r, bx.n4_div), i4 = r - sc * n4, s = sc / 3, c = sc - 3 * s;
-                        src = b + cb<FR>(c) * N + (long)(t.j0 + s) * f.sj + t.kb + i4 * VE;
-                        dst[q] = sb + (s * 3 + c) * n + i4 * VE;
-                    } else if (e < eN) {
-                        const int i4 = e - eB;
-                        src = b + Comp<FR>::b[1] * N + (long)jn * f.sj + t.kb + i4 * VE;
-                        dst[q] = sb2n + i4 * VE;
-                    } else {
-                        const int r = e - eN, s = fdiv(r, bx.n4_div), i4 = r - s * n4;
-                        src = b + Comp<FR>::b[2] * N + (long)(t.j0 + s) * f.sj + t.kpb + i4 * VE;
-                        dst[q] = sbk + s * n + i4 * VE;
-                    }
-                    v[q] = *reinterpret_cast<const V16<T> *>(src);
-                }
+        auto item = [&](int e, int &doff) -> const T * {
+            if (e < eU) {
+                const int sc = fdiv(e, bx.n4_div), i4 = e - sc * n4, s = sc / 5, c = sc - 5 * s;
+                doff = (s * 5 + c) * n + i4 * VE;
+                return u + cu<FR>(c) * N + (long)(t.j0 + s) * f.sj + t.kb + i4 * VE;
+            } else if (e < eB) {
+                const int r = e - eU, sc = fdiv(r, bx.n4_div), i4 = r - sc * n4, s = sc / 3, c = sc - 3 * s;
+                doff = 5 * W * n + (s * 3 + c) * n + i4 * VE;
+                return b + cb<FR>(c) * N + (long)(t.j0 + s) * f.sj + t.kb + i4 * VE;
+            } else if (e < eN) {
+                const int i4 = e - eB;
+                doff = 8 * W * n + i4 * VE;
+                return b + Comp<FR>::b[1] * N + (long)jn * f.sj + t.kb + i4 * VE;
+            } else {
+                const int r = e - eN, s = fdiv(r, bx.n4_div), i4 = r - s * n4;
+                doff = 8 * W * n + n + s * n + i4 * VE;
+                return b + Comp<FR>::b[2] * N + (long)(t.j0 + s) * f.sj + t.kpb + i4 * VE;
             }
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-                if (dst[q]) *reinterpret_cast<V16<T> *>(dst[q]) = v[q];
+        };
+        for (int f0 = tid; f0 < eK; f0 += 4 * nt) {
+            int d0 = -1, d1 = -1, d2 = -1, d3 = -1;
+            V16<T> v0, v1, v2, v3;
+            if (f0 < eK) v0 = *reinterpret_cast<const V16<T> *>(item(f0, d0));
+            if (f0 + nt < eK) v1 = *reinterpret_cast<const V16<T> *>(item(f0 + nt, d1));
+            if (f0 + 2 * nt < eK) v2 = *reinterpret_cast<const V16<T> *>(item(f0 + 2 * nt, d2));
+            if (f0 + 3 * nt < eK) v3 = *reinterpret_cast<const V16<T> *>(item(f0 + 3 * nt, d3));
+            if (d0 >= 0) *reinterpret_cast<V16<T> *>(smem + d0) = v0;
+            if (d1 >= 0) *reinterpret_cast<V16<T> *>(smem + d1) = v1;
+            if (d2 >= 0) *reinterpret_cast<V16<T> *>(smem + d2) = v2;
+            if (d3 >= 0) *reinterpret_cast<V16<T> *>(smem + d3) = v3;
         }
-    } else if (t.vec) {
+    } else if (FR != 0 && t.vec) {
         // y/z frames, full tile: items (segment, component, row i); a row of
         // the tile is one vector spanning the W slots
         const int eU = 5 * n, eB = eU + 3 * n, eK = eB + n;
@@ -244,14 +243,14 @@ __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__re
     __syncthreads();
 
     // store the new u of the tile's pencils
-    if (t.vec && f.si == 1) {
+    if (FR == 0 && t.vec) {
         const int n4 = n / VE, eU = ns * 5 * n4;
         for (int e = tid; e < eU; e += nt) {
             const int sc = fdiv(e, bx.n4_div), i4 = e - sc * n4, s = sc / 5, c = sc - 5 * s;
             *reinterpret_cast<V16<T> *>(u + cu<FR>(c) * N + (long)(t.j0 + s) * f.sj + t.kb + i4 * VE) =
                 *reinterpret_cast<const V16<T> *>(su + (s * 5 + c) * n + i4 * VE);
         }
-    } else if (t.vec) {
+    } else if (FR != 0 && t.vec) {
         for (int e = tid; e < 5 * n; e += nt) {
             const int c = fdiv(e, bx.n_div), i = e - c * n;
             V16<T> v;
@@ -300,7 +299,7 @@ __device__ __forceinline__ void advect_soa_tile(const T *__restrict__ u, const T
     const T *rho = u, *mom = u + Comp<FR>::u[1] * N;
     const int nrow = 3 * ns + 1;
 
-    if (t.vec && f.si == 1) {
+    if (FR == 0 && t.vec) {
         const int n4 = n / VE;
         // face velocities: rows x vectors
         for (int e = tid; e < nrow * n4; e += nt) {
@@ -333,36 +332,34 @@ __device__ __forceinline__ void advect_soa_tile(const T *__restrict__ u, const T
         }
         // b of the tile (3 comps), b2 after the tile, b3 at k+1
         const int eB = ns * 3 * n4, eN = eB + n4, eK = eN + ns * n4;
-        for (int f0 = tid; f0 < eK; f0 += 4 * nt) {
-            V16<T> v[4];
-            T *dst[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int e = f0 + q * nt;
-                dst[q] = nullptr;
-                if (e < eK) {
-                    const T *src;
-                    if (e < eB) {
-                        const int sc = fdiv(e, bx.n4_div), i4 = e - sc * n4, s = sc / 3, c = sc - 3 * s;
-                        src = bin + cb<FR>(c) * N + (long)(t.j0 + s) * f.sj + t.kb + i4 * VE;
-                        dst[q] = sb + (s * 3 + c) * n + i4 * VE;
-                    } else if (e < eN) {
-                        const int i4 = e - eB;
-                        src = bin + Comp<FR>::b[1] * N + (long)jn * f.sj + t.kb + i4 * VE;
-                        dst[q] = sb2n + i4 * VE;
-                    } else {
-                        const int r = e - eN, s = fdiv(r, bx.n4_div), i4 = r - s * n4;
-                        src = bin + Comp<FR>::b[2] * N + (long)(t.j0 + s) * f.sj + t.kpb + i4 * VE;
-                        dst[q] = sb3p + s * n + i4 * VE;
-                    }
-                    v[q] = *reinterpret_cast<const V16<T> *>(src);
-                }
+        auto item = [&](int e, int &doff) -> const T * {
+            if (e < eB) {
+                const int sc = fdiv(e, bx.n4_div), i4 = e - sc * n4, s = sc / 3, c = sc - 3 * s;
+                doff = (3 * W + 1) * n + (s * 3 + c) * n + i4 * VE;
+                return bin + cb<FR>(c) * N + (long)(t.j0 + s) * f.sj + t.kb + i4 * VE;
+            } else if (e < eN) {
+                const int i4 = e - eB;
+                doff = (6 * W + 1) * n + i4 * VE;
+                return bin + Comp<FR>::b[1] * N + (long)jn * f.sj + t.kb + i4 * VE;
+            } else {
+                const int r = e - eN, s = fdiv(r, bx.n4_div), i4 = r - s * n4;
+                doff = (6 * W + 2) * n + s * n + i4 * VE;
+                return bin + Comp<FR>::b[2] * N + (long)(t.j0 + s) * f.sj + t.kpb + i4 * VE;
             }
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-                if (dst[q]) *reinterpret_cast<V16<T> *>(dst[q]) = v[q];
+        };
+        for (int f0 = tid; f0 < eK; f0 += 4 * nt) {
+            int d0 = -1, d1 = -1, d2 = -1, d3 = -1;
+            V16<T> v0, v1, v2, v3;
+            if (f0 < eK) v0 = *reinterpret_cast<const V16<T> *>(item(f0, d0));
+            if (f0 + nt < eK) v1 = *reinterpret_cast<const V16<T> *>(item(f0 + nt, d1));
+            if (f0 + 2 * nt < eK) v2 = *reinterpret_cast<const V16<T> *>(item(f0 + 2 * nt, d2));
+            if (f0 + 3 * nt < eK) v3 = *reinterpret_cast<const V16<T> *>(item(f0 + 3 * nt, d3));
+            if (d0 >= 0) *reinterpret_cast<V16<T> *>(smem + d0) = v0;
+            if (d1 >= 0) *reinterpret_cast<V16<T> *>(smem + d1) = v1;
+            if (d2 >= 0) *reinterpret_cast<V16<T> *>(smem + d2) = v2;
+            if (d3 >= 0) *reinterpret_cast<V16<T> *>(smem + d3) = v3;
         }
-    } else if (t.vec) {
+    } else if (FR != 0 && t.vec) {
         // y/z frames, full tile: one item per row i does every face velocity
         for (int i = tid; i < n; i += nt) {
             const long ro = (long)i * f.si;
@@ -515,14 +512,14 @@ __device__ __forceinline__ void advect_soa_tile(const T *__restrict__ u, const T
     }
     __syncthreads();
 
-    if (t.vec && f.si == 1) {
+    if (FR == 0 && t.vec) {
         const int n4 = n / VE, eB = ns * 3 * n4;
         for (int e = tid; e < eB; e += nt) {
             const int sc = fdiv(e, bx.n4_div), i4 = e - sc * n4, s = sc / 3, c = sc - 3 * s;
             *reinterpret_cast<V16<T> *>(bout + cb<FR>(c) * N + (long)(t.j0 + s) * f.sj + t.kb + i4 * VE) =
                 *reinterpret_cast<const V16<T> *>(sb + (s * 3 + c) * n + i4 * VE);
         }
-    } else if (t.vec) {
+    } else if (FR != 0 && t.vec) {
         for (int e = tid; e < 3 * n; e += nt) {
             const int c = fdiv(e, bx.n_div), i = e - c * n;
             V16<T> v;
