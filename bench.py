@@ -37,15 +37,35 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def pmc_traffic(kernel="fluidx", n=256):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_{n}cube_pmc.txt")
+def profile_value(fname, key):
+    """A `key: value` line of a committed rocprofv3 summary under profiles/, if any."""
+    path = os.path.join(ROOT, "profiles", fname)
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":")[1])
     return None
+
+
+def pmc_traffic(kernel="fluidx", n=256):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+    return profile_value(f"{kernel}_{n}cube_pmc.txt", "traffic_bytes_per_launch")
+
+
+def valu_roofline(fl_ms, n):
+    """fluidx is limited by VALU issue, not HBM: its VALU operations per cell
+    (rocprofv3 SQ counters, committed summary) at 4 cycles per wave64
+    operation (16 for a transcendental) on 256 CUs x 4 SIMDs at 2.4 GHz give
+    the time of a launch at 100% VALU issue."""
+    vpc = profile_value(f"sweeps_{n}cube_sq.txt", "valu_per_cell_fluidx")
+    tpc = profile_value(f"sweeps_{n}cube_sq.txt", "trans_per_cell_fluidx")
+    if vpc is None or tpc is None:
+        return None
+    cyc = vpc / 64 * 4 + tpc / 64 * 12
+    floor_ms = n ** 3 * cyc / (256 * 4) / 2.4e9 * 1e3
+    return {"kernel": "fluidx", "bound": "valu", "valu_ops_per_cell": vpc, "simd_cycles_per_cell": cyc,
+            "floor_ms": floor_ms, "avg_launch_ms": fl_ms, "frac": floor_ms / fl_ms}
 
 
 def cpu_baseline(u0, b0, n, seconds_hint=30.0):
@@ -252,6 +272,7 @@ def main():
         "roofline": {"kernel": "fluidx", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic("fluidx", n),
                      "bytes_per_launch": FLUIDX_BYTES_PER_CELL * cells, "avg_launch_ms": fl_ms},
+        "valu_roofline": valu_roofline(fl_ms, n),
         "step_roofline": {"bytes_per_zone_update": STEP_BYTES_PER_CELL,
                           "achieved_GBs": zus * STEP_BYTES_PER_CELL / 1e9,
                           "frac": zus * STEP_BYTES_PER_CELL / 1e9 / HBM_PEAK_GBS},
