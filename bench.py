@@ -56,13 +56,12 @@ def pmc_traffic(kernel="fluidx", n=256):
 def valu_roofline(fl_ms, n):
     """fluidx is limited by VALU issue, not HBM: its VALU operations per cell
     (rocprofv3 SQ counters, committed summary) at 4 cycles per wave64
-    operation (16 for a transcendental) on 256 CUs x 4 SIMDs at 2.4 GHz give
-    the time of a launch at 100% VALU issue."""
+    operation on 256 CUs x 4 SIMDs at 2.4 GHz give a lower bound on the time
+    of a launch (100% VALU issue, transcendentals counted at full rate)."""
     vpc = profile_value(f"sweeps_{n}cube_sq.txt", "valu_per_cell_fluidx")
-    tpc = profile_value(f"sweeps_{n}cube_sq.txt", "trans_per_cell_fluidx")
-    if vpc is None or tpc is None:
+    if vpc is None:
         return None
-    cyc = vpc / 64 * 4 + tpc / 64 * 12
+    cyc = vpc / 64 * 4
     floor_ms = n ** 3 * cyc / (256 * 4) / 2.4e9 * 1e3
     return {"kernel": "fluidx", "bound": "valu", "valu_ops_per_cell": vpc, "simd_cycles_per_cell": cyc,
             "floor_ms": floor_ms, "avg_launch_ms": fl_ms, "frac": floor_ms / fl_ms}
