@@ -188,6 +188,65 @@ def bench_served(mhdgpu, n_req=64):
             "throughput_ratio": batched["zone_updates_per_s"] / fifo["zone_updates_per_s"]}
 
 
+VARS = ["rho", "mx", "my", "mz", "E", "bx", "by", "bz"]
+
+
+def precision_report(mhdgpu, n, steps, ic, seed):
+    """FP32 against FP64 on the device from the same initial state: relative
+    L2 error per variable after `steps` steps, FP64 drift of the total mass
+    and energy, max |div B| (left faces), and the dt of both runs."""
+    from mhdgpu.ics import random_phase, turbulence
+    log(f"precision: generating {ic} {n}^3 (seed {seed})")
+    if ic == "turbulence":
+        u0, b0 = turbulence(n, seed=seed, progress=log)
+    else:
+        u0, b0 = random_phase(n, seed=seed)
+    runs = {}
+    for prec in (8, 4):
+        name = "fp64" if prec == 8 else "fp32"
+        sim = mhdgpu.Simulation(n, n, n, precision=prec)
+        sim.set_state(u0, b0)
+        m0 = sim.monitors()
+        log(f"precision: {name} {steps} steps")
+        t0 = time.perf_counter()
+        done = 0
+        while done < steps:
+            k = min(50, steps - done)
+            sim.step(k)
+            sim.sync()
+            done += k
+            log(f"precision: {name} {done}/{steps}")
+        wall = time.perf_counter() - t0
+        m1 = sim.monitors()
+        d = sim.dts()
+        u, b = sim.get_state()
+        t, t64 = sim.time()
+        sim.close()
+        runs[name] = {"u": u, "b": b, "m0": m0, "m1": m1, "dts": d, "ms_per_step": wall * 1e3 / steps,
+                      "t": t64}
+    r64, r32 = runs["fp64"], runs["fp32"]
+    rel = {}
+    for c in range(5):
+        a, e = r32["u"][c::5].astype(np.float64), r64["u"][c::5]
+        rel[VARS[c]] = float(np.linalg.norm(a - e) / np.linalg.norm(e))
+    for c in range(3):
+        a, e = r32["b"][c::3].astype(np.float64), r64["b"][c::3]
+        rel[VARS[5 + c]] = float(np.linalg.norm(a - e) / np.linalg.norm(e))
+    allv = (sum(float(np.sum((r32["u"][c::5] - r64["u"][c::5]) ** 2)) for c in range(5)) +
+            sum(float(np.sum((r32["b"][c::3] - r64["b"][c::3]) ** 2)) for c in range(3)))
+    alln = float(np.sum(r64["u"] ** 2) + np.sum(r64["b"] ** 2))
+    out = {"mode": "precision", "ic": ic, "grid": [n, n, n], "steps": steps, "seed": seed,
+           "rel_l2_fp32_vs_fp64": rel, "rel_l2_all": (allv / alln) ** 0.5}
+    for name, r in runs.items():
+        out[name] = {"mass_drift": float((r["m1"][0] - r["m0"][0]) / r["m0"][0]),
+                     "energy_drift": float((r["m1"][1] - r["m0"][1]) / r["m0"][1]),
+                     "max_divB_start": float(r["m0"][5]), "max_divB_end": float(r["m1"][5]),
+                     "t_end": r["t"], "dt_first": float(r["dts"][0]), "dt_last": float(r["dts"][-1]),
+                     "finite": bool(np.isfinite(r["u"]).all() and np.isfinite(r["b"]).all()),
+                     "ms_per_step": r["ms_per_step"]}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,8 +257,11 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--requests", type=int, default=64)
-    ap.add_argument("--mode", choices=["plain", "copy"], default="plain",
-                    help="copy: time the plain copy kernel on 1 GiB (counter calibration)")
+    ap.add_argument("--mode", choices=["plain", "copy", "precision"], default="plain",
+                    help="copy: time the plain copy kernel on 1 GiB (counter calibration); "
+                         "precision: FP32 vs FP64 report on --ic at --n for --steps")
+    ap.add_argument("--ic", choices=["random_phase", "turbulence"], default="random_phase")
+    ap.add_argument("--seed", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it", file=sys.stderr)
@@ -219,6 +281,10 @@ def main():
         wall = time.perf_counter() - t0
         print(json.dumps({"mode": "copy", "bytes_copied": nbytes, "launches": args.steps,
                           "GBs_read_plus_write": 2 * nbytes * args.steps / wall / 1e9}), flush=True)
+        return
+
+    if args.mode == "precision":
+        print(json.dumps(precision_report(mhdgpu, args.n, args.steps, args.ic, args.seed)), flush=True)
         return
 
     n = args.n
@@ -279,6 +345,19 @@ def main():
                       "sweeps": {f"{i:02d}_{s}": float(v) for i, (s, v) in enumerate(zip(SWEEPS, sweep_ms))}},
         "monitors_fp64": {"mass": m[0], "energy": m[1], "max_divB": m[5], "t": t, "t64": t64, "finite": finite},
     }
+    # the drop-in program loop on host buffers (cuda_main's contract): the
+    # rate includes device allocation, the H2D copy of u and b, the steps and
+    # the D2H copy back; a first 1-step call loads the code objects
+    uh, bh = u0.copy(), b0.copy()
+    mhdgpu.main(uh, bh, n, n, n, 1)
+    uh, bh = u0.copy(), b0.copy()
+    t0 = time.perf_counter()
+    mhdgpu.main(uh, bh, n, n, n, args.steps)
+    wall_main = time.perf_counter() - t0
+    out["drop_in_main"] = {"steps": args.steps, "wall_ms": wall_main * 1e3,
+                           "zone_updates_per_s": n ** 3 * args.steps / wall_main,
+                           "includes": "device allocation, H2D of u and b, the steps, D2H (PCIe-inclusive)"}
+    del uh, bh
     if not args.no_batched:
         out["batched"] = bench_batched(mhdgpu)
     if not args.no_served:

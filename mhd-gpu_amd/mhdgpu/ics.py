@@ -18,9 +18,22 @@ b: 3 per cell on the left faces, x fastest), periodic, gamma = 5/3.
   (sigma 0.5, mean 1), solenoidal velocity P(k) ~ k^-11/3 for 2 <= k <= 32 at
   rms Mach 1, B = curl A with plasma beta 2 p0/<B^2> = 1.
 """
+import os
+
 import numpy as np
 
+try:
+    import scipy.fft as _fft
+    _FFT_KW = {"workers": min(16, os.cpu_count() or 1)}
+except ImportError:  # numpy's single-threaded FFT
+    _fft = np.fft
+    _FFT_KW = {}
+
 GAMMA = 5.0 / 3.0
+
+
+def _noop(*a):
+    pass
 
 
 def _pack(rho, v, bface, p):
@@ -92,7 +105,7 @@ def _gaussian_field(n, rng, kmin, kmax, slope, ncomp=3, solenoidal=False):
         k2 = np.where(kk > 0, kk ** 2, 1.0)
         kd = (kx * fh[0] + ky * fh[1] + kz * fh[2]) / k2
         fh = [fh[0] - kx * kd, fh[1] - ky * kd, fh[2] - kz * kd]
-    return np.stack([np.real(np.fft.ifftn(f)) for f in fh])
+    return np.stack([np.real(_fft.ifftn(f, **_FFT_KW)) for f in fh])
 
 
 def _curl_faces(A):
@@ -124,16 +137,20 @@ def random_phase(n, seed, mach=0.1, db=0.1, p0=0.6):
     return _pack(np.ones(shape), v, bf, np.full(shape, p0))
 
 
-def turbulence(n, seed, mach=1.0, beta=1.0, sigma=0.5, p0=0.6):
+def turbulence(n, seed, mach=1.0, beta=1.0, sigma=0.5, p0=0.6, progress=_noop):
     rng = np.random.default_rng(seed)
+    progress("turbulence: density")
     g = _gaussian_field(n, rng, 1, n // 4, -11.0 / 3.0, ncomp=1)[0]
     g = g / g.std()
     rho = np.exp(sigma * g)
     rho /= rho.mean()
+    progress("turbulence: velocity")
     v = _gaussian_field(n, rng, 2, min(32, n // 2), -11.0 / 3.0, solenoidal=True)
     cs = np.sqrt(GAMMA * p0)
     v *= mach * cs / _rms(v)
+    progress("turbulence: vector potential")
     A = _gaussian_field(n, rng, 2, min(32, n // 2), -11.0 / 3.0)
+    progress("turbulence: field")
     bf = _curl_faces(A)
     bf *= np.sqrt(2 * p0 / beta) / _rms(bf)
     return _pack(rho, v, bf, np.full(rho.shape, p0))
