@@ -173,7 +173,8 @@ def run_served(mhdgpu, mode, sizes, steps, arrivals, states):
     return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
             "mean_s": float(lat.mean()), "zone_updates": st["zone_updates"], "span_s": st["span_s"],
             "zone_updates_per_s": st["zone_updates"] / st["span_s"], "gpu_busy_s": st["gpu_busy_s"],
-            "rebuilds": int(st["rebuilds"]), "all_ok": ok}
+            "rebuilds": int(st["rebuilds"]), "all_ok": ok,
+            "service_thread_s": {k[2:-2]: st[k] for k in ("t_upload_s", "t_rebuild_s", "t_step_s", "t_download_s")}}
 
 
 def bench_served(mhdgpu, n_req=64):

@@ -159,9 +159,11 @@ int mhd_ctx_profile_step(mhd_ctx *ctx, int reps, float *ms_out13);
  * No counterpart in the reference (one compile-time box per program,
  * Heterogeneous_MHD/C/parameter.h:6).  Each box advances with its own dt
  * exactly as a lone context would (results are bit-identical to
- * mhd_ctx_step on the same box); one step of the batch is one CFL launch,
- * one dt launch and 12 persistent sweep kernels over all boxes, captured in
- * a hipGraph.  FP32.  dims holds nx,ny,nz of each box. */
+ * mhd_ctx_step on the same box).  One step of the batch is captured in a
+ * hipGraph: under the default schedule the boxes are grouped by their
+ * cells-per-lane K, and each group runs one CFL launch, one dt launch and
+ * 12 sweep kernels over all its boxes, the groups concurrently.  FP32.
+ * dims holds nx,ny,nz of each box. */
 typedef struct mhd_batch mhd_batch;
 int mhd_batch_create(mhd_batch **bt, int nbox, const int *dims, int avg_b);
 int mhd_batch_destroy(mhd_batch *bt);
@@ -173,7 +175,9 @@ int mhd_batch_sync(mhd_batch *bt);
 int mhd_batch_dts(mhd_batch *bt, int box, double *h_dts, int max, int *count);
 /* schedule of the sweeps: blocks per CU of the persistent kernels (0 =
  * occupancy limit) and policy (0 = persistent, one queue per XCD with
- * stealing; 1 = persistent, one global queue; 2 = one block per work item) */
+ * stealing; 1 = persistent, one global queue; 2 = one kernel per sweep over
+ * all boxes, one block per work item; 3 = the default: boxes grouped by K,
+ * one exact-K kernel per sweep and group, the groups' chains concurrent) */
 int mhd_batch_set_schedule(mhd_batch *bt, int blocks_per_cu, int policy);
 /* times the candidate schedules on the batch itself (trial_steps each),
  * restores every box's state and clock, keeps the fastest */
@@ -202,6 +206,11 @@ int mhd_server_pending(mhd_server *sv, int *queued, int *active);
 /* out[7]: zone-updates, seconds first arrival -> last completion, GPU busy
  * seconds, batched steps, work-list rebuilds, bytes resident, budget */
 int mhd_server_stats(mhd_server *sv, double *out7);
+/* out[4]: seconds the service thread spent uploading admitted requests,
+ * rebuilding the work lists and the step graph, stepping (launch and wait)
+ * and reading finished requests back (continuous batching; zeros for FIFO).
+ * No counterpart in the reference (its driver has no request loop). */
+int mhd_server_timing(mhd_server *sv, double *out4);
 /* finishes every submitted request, then stops the service thread */
 int mhd_server_destroy(mhd_server *sv);
 

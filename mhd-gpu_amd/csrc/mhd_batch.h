@@ -1,10 +1,14 @@
 // Batched engine: many independent boxes of uneven sizes advanced together,
-// each with its own dt.  One step is 1 CFL launch over all boxes, 1 dt
-// launch, and 12 sweeps, each ONE persistent kernel that pulls (box, tile)
-// work items from per-XCD queues (own queue first, then stealing).  The step
-// is captured in a hipGraph and replayed.  Boxes live in slots; the active
-// set can change between steps (the server admits and retires requests that
-// way).  FP32 only, the reference's precision.
+// each with its own dt.  One step is a CFL launch, a dt launch and 12 sweep
+// launches over (box, tile) work items, captured in a hipGraph and replayed.
+// Schedules: the default groups the boxes by cells-per-lane K so that each
+// sweep kernel is compiled for one K (and holds only its registers), and
+// runs the groups' chains as concurrent branches of the graph; the others
+// run one kernel per sweep over all boxes, either one block per item or
+// persistent blocks pulling items from per-XCD queues (own queue first,
+// then stealing).  Boxes live in slots; the active set can change between
+// steps (the server admits and retires requests that way).  FP32 only, the
+// reference's precision.
 #pragma once
 #include <vector>
 
@@ -24,7 +28,19 @@ struct WorkItem {
     int box, tile;
 };
 struct CflItem {
-    int box, row0, nrows;
+    int box, e0, ne;  // units e0 .. e0+ne-1 of the box (see cfl_range)
+};
+
+// boxes with the same cells-per-lane K in every frame (schedule 3): their
+// items, CFL items and active slots are contiguous ranges of the engine's
+// lists, and they run exact-K kernels
+struct BoxGroup {
+    int K[3];
+    int item_off[3], item_n[3];
+    int cfl_off, cfl_n;
+    int act_off, act_n;
+    int lds_f[3], lds_a[3];  // dynamic LDS bytes of fluidx / advect per frame
+    long cells;
 };
 
 class Engine {
@@ -50,13 +66,17 @@ public:
     // the set of slots a step advances (rebuilds work lists; re-captures)
     int set_active(const std::vector<int> &slots);
     const std::vector<int> &active() const { return active_; }
+    // rebuild the work lists and re-capture the step if the active set changed
+    int prepare();
     int step(int nsteps);
     int step_timed(int nsteps, float *ms);
     int sync();
 
     // schedule: persistent blocks per CU (0 = occupancy limit) and policy
     // (0: persistent, one queue per XCD + stealing; 1: persistent, one
-    // global queue; 2: one block per work item)
+    // global queue; 2: one block per work item; 3: boxes grouped by K, each
+    // group a chain of exact-K kernels, one block per item, the groups'
+    // chains concurrent branches of the step graph)
     int set_schedule(int blocks_per_cu, int policy);
     int autotune(int trial_steps, int *bpc, int *policy, float *best_ms);
 
@@ -85,8 +105,12 @@ private:
     WorkItem *d_items_[3] = {nullptr, nullptr, nullptr};
     CflItem *d_cfl_ = nullptr;
     int nitems_[3] = {0, 0, 0}, ncfl_ = 0, cap_items_[3] = {0, 0, 0}, cap_cfl_ = 0;
+    std::vector<BoxGroup> groups_;
+    static constexpr int kAux = 4;  // side streams of the grouped schedule
+    hipStream_t aux_[kAux] = {};
+    hipEvent_t evfork_ = nullptr, evjoin_[kAux] = {};
     int kmax_ = 1, lds_fluidx_ = 0, lds_advect_ = 0;
-    int bpc_ = 0, policy_ = 2, grid_ = 0, ncu_ = 0;  // default: one block per item (fastest measured)
+    int bpc_ = 0, policy_ = 3, grid_ = 0, ncu_ = 0;  // default: K-grouped concurrent chains (fastest measured)
     bool dirty_ = true;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;

@@ -132,7 +132,27 @@ __global__ void __launch_bounds__(64 * kBW)
 #undef MHD_GITEM
 }
 
-// CFL over the active boxes: one block per (box, row range); per-box max
+// One K for every item (schedule 3): the kernel holds only the registers of
+// that K, so small-K boxes keep their occupancy beside large-K ones.
+template <int FR, int K, bool ADV>
+__global__ void __launch_bounds__(64 * kBW)
+    batch_group_kernel(const BoxDesc *__restrict__ boxes, const WorkItem *__restrict__ items,
+                       const float *__restrict__ dt, int flip)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float *smem = reinterpret_cast<float *>(smem_raw);
+    const WorkItem wi = items[xcd_tile(blockIdx.x, gridDim.x)];
+    const BoxDesc &bd = boxes[wi.box];
+    float *const u = bd.u;
+    const float *bcur = flip ? bd.b2 : bd.b;
+    float *bnext = flip ? bd.b : bd.b2;
+    if constexpr (ADV)
+        advect_soa_tile<float, K, kBW, FR>(u, bcur, bnext, bd.fr[FR], dt + wi.box, wi.tile, bd.ntj[FR], smem);
+    else
+        fluidx_soa_tile<float, K, kBW, FR>(u, bcur, bd.fr[FR], dt + wi.box, wi.tile, bd.ntj[FR], smem);
+}
+
+// CFL over the active boxes: one block per (box, unit range); per-box max
 template <bool AVG>
 __global__ void __launch_bounds__(256) batch_cfl_kernel(const BoxDesc *__restrict__ boxes,
                                                         const CflItem *__restrict__ items,
@@ -140,32 +160,7 @@ __global__ void __launch_bounds__(256) batch_cfl_kernel(const BoxDesc *__restric
 {
     const CflItem it = items[blockIdx.x];
     const BoxDesc &bd = boxes[it.box];
-    const SoaGrid g = bd.g;
-    const float *u = bd.u, *b = bd.b;
-    const long N = g.N;
-    float c = 0.f;
-    for (int r = it.row0; r < it.row0 + it.nrows; r++) {
-        const int j = r % g.ny, k = r / g.ny;
-        const long row = (long)j * g.sy + (long)k * g.sz;
-        for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
-            const long s = row + i;
-            const float b1 = b[s], b2 = b[N + s], b3 = b[2 * N + s];
-            float B;
-            if constexpr (AVG) {
-                const long sip = row + (i + 1 == g.nx ? 0 : i + 1);
-                const long sjp = (long)(j + 1 == g.ny ? 0 : j + 1) * g.sy + (long)k * g.sz + i;
-                const long skp = (long)j * g.sy + (long)(k + 1 == g.nz ? 0 : k + 1) * g.sz + i;
-                const float bxa = (float)((double)(b1 + b[sip]) / 2.0);
-                const float bya = (float)((double)(b2 + b[N + sjp]) / 2.0);
-                const float bza = (float)((double)(b3 + b[2 * N + skp]) / 2.0);
-                B = bxa * bxa + bya * bya + bza * bza;
-            } else {
-                B = b1 * b1 + b2 * b2 + b3 * b3;
-            }
-            const float sp = cfl_speed<float>(u[s], u[N + s], u[2 * N + s], u[3 * N + s], u[4 * N + s], B);
-            c = c > sp ? c : sp;
-        }
-    }
+    float c = cfl_range<float, AVG>(bd.u, bd.b, bd.g, (long)it.e0 + threadIdx.x, (long)it.e0 + it.ne, blockDim.x);
     c = wave_max(c);
     __shared__ float red[4];
     if (lane_id() == 0) red[threadIdx.x >> 6] = c;
@@ -251,6 +246,40 @@ static hipError_t launch_grid(int fr, int kmax, bool adv, int nitems, size_t lds
     return hipGetLastError();
 }
 
+template <int FR, int K> static const void *group_ptr(bool adv)
+{
+    return adv ? (const void *)batch_group_kernel<FR, K, true> : (const void *)batch_group_kernel<FR, K, false>;
+}
+
+static const void *group_kernel(int fr, int K, bool adv)
+{
+#define MHD_GK(F)                                                                                                    \
+    if (fr == F) switch (K) {                                                                                        \
+        case 1: return group_ptr<F, 1>(adv);                                                                         \
+        case 2: return group_ptr<F, 2>(adv);                                                                         \
+        case 3: return group_ptr<F, 3>(adv);                                                                         \
+        case 4: return group_ptr<F, 4>(adv);                                                                         \
+        case 5: return group_ptr<F, 5>(adv);                                                                         \
+        case 6: return group_ptr<F, 6>(adv);                                                                         \
+        case 8: return group_ptr<F, 8>(adv);                                                                         \
+        default: return nullptr;                                                                                     \
+        }
+    MHD_GK(0)
+    MHD_GK(1)
+    MHD_GK(2)
+#undef MHD_GK
+    return nullptr;
+}
+
+static hipError_t launch_group(int fr, int K, bool adv, int nitems, size_t lds, hipStream_t st,
+                               const BoxDesc *boxes, const WorkItem *items, const float *dt, int flip)
+{
+    const void *k = group_kernel(fr, K, adv);
+    if (!k) return hipErrorInvalidValue;
+    void *args[] = {(void *)&boxes, (void *)&items, (void *)&dt, (void *)&flip};
+    return hipLaunchKernel(k, dim3(nitems), dim3(64 * kBW), args, lds, st);
+}
+
 static const void *sweep_kernel(int fr, int kmax, bool adv)
 {
 #define MHD_KP(F)                                                                                                    \
@@ -311,6 +340,14 @@ Engine::~Engine()
         if (p) (void)hipFree(p);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
+    if (evfork_) (void)hipEventDestroy(evfork_);
+    for (int i = 0; i < kAux; i++) {
+        if (evjoin_[i]) (void)hipEventDestroy(evjoin_[i]);
+        if (aux_[i]) {
+            (void)hipStreamSynchronize(aux_[i]);
+            (void)hipStreamDestroy(aux_[i]);
+        }
+    }
     if (st_) (void)hipStreamDestroy(st_);
 }
 
@@ -358,6 +395,20 @@ int Engine::init()
                                             128 * 1024);
                 if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute");
             }
+    for (int fr = 0; fr < 3 && e == hipSuccess; fr++)
+        for (int K : {1, 2, 3, 4, 5, 6, 8})
+            for (int adv = 0; adv < 2 && e == hipSuccess; adv++)
+                e = hipFuncSetAttribute(group_kernel(fr, K, adv), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        128 * 1024);
+    if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute");
+    // side streams (default priority) and the fork/join events of the
+    // grouped schedule's step graph
+    for (int i = 0; i < kAux && e == hipSuccess; i++) {
+        e = hipStreamCreate(&aux_[i]);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&evjoin_[i], hipEventDisableTiming);
+    }
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&evfork_, hipEventDisableTiming);
+    if (e != hipSuccess) return hip_fail(e, "engine side streams");
     e = hipStreamSynchronize(st_);
     return e == hipSuccess ? MHD_OK : hip_fail(e, "engine init");
 }
@@ -524,7 +575,7 @@ int Engine::set_active(const std::vector<int> &slots)
 
 int Engine::set_schedule(int bpc, int policy)
 {
-    if (bpc < 0 || policy < 0 || policy > 2) return fail(MHD_ERR_ARG, "set_schedule: bad argument");
+    if (bpc < 0 || policy < 0 || policy > 3) return fail(MHD_ERR_ARG, "set_schedule: bad argument");
     bpc_ = bpc;
     policy_ = policy;
     dirty_ = true;
@@ -550,23 +601,70 @@ int Engine::build()
     // largest boxes first: the queues then hand out the longest items first
     std::vector<int> order = active_;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cells(a) > cells(b); });
+    // schedule 3: boxes grouped by their K in every frame, the groups with
+    // the most cells first; the lists below then hold each group contiguously
+    groups_.clear();
+    if (policy_ == 3) {
+        std::vector<std::vector<int>> members;
+        for (int s : order) {
+            size_t g = 0;
+            for (; g < groups_.size(); g++)
+                if (groups_[g].K[0] == boxes_[s].K[0] && groups_[g].K[1] == boxes_[s].K[1] &&
+                    groups_[g].K[2] == boxes_[s].K[2])
+                    break;
+            if (g == groups_.size()) {
+                BoxGroup G;
+                memset(&G, 0, sizeof G);
+                for (int fr = 0; fr < 3; fr++) G.K[fr] = boxes_[s].K[fr];
+                groups_.push_back(G);
+                members.emplace_back();
+            }
+            members[g].push_back(s);
+            groups_[g].cells += cells(s);
+        }
+        std::vector<int> gi(groups_.size());
+        for (size_t g = 0; g < gi.size(); g++) gi[g] = (int)g;
+        std::stable_sort(gi.begin(), gi.end(), [&](int a, int b) { return groups_[a].cells > groups_[b].cells; });
+        std::vector<BoxGroup> sorted;
+        order.clear();
+        for (int g : gi) {
+            BoxGroup G = groups_[g];
+            G.act_off = (int)order.size();
+            G.act_n = (int)members[g].size();
+            for (int s : members[g]) order.push_back(s);
+            sorted.push_back(G);
+        }
+        groups_.swap(sorted);
+    }
     kmax_ = 1;
     lds_fluidx_ = lds_advect_ = 0;
     std::vector<WorkItem> items[3];
     std::vector<int> part(3 * (kQueues + 1), 0);
     for (int fr = 0; fr < 3; fr++) {
         std::vector<double> cost;
-        for (int s : order) {
+        size_t gnext = 0;
+        for (size_t oi = 0; oi < order.size(); oi++) {
+            const int s = order[oi];
+            if (gnext < groups_.size() && (int)oi == groups_[gnext].act_off) groups_[gnext++].item_off[fr] = (int)items[fr].size();
             const BoxDesc &d = boxes_[s];
             const int nt = d.ntj[fr] * d.fr[fr].f.nk;
             kmax_ = std::max(kmax_, d.K[fr]);
-            lds_fluidx_ = std::max(lds_fluidx_, (int)(4 * (9 * kBW + 1) * d.fr[fr].f.ni));
-            lds_advect_ = std::max(lds_advect_, (int)(4 * (7 * kBW + 2) * d.fr[fr].f.ni));
+            const int lf = (int)(4 * (9 * kBW + 1) * d.fr[fr].f.ni), la = (int)(4 * (7 * kBW + 2) * d.fr[fr].f.ni);
+            lds_fluidx_ = std::max(lds_fluidx_, lf);
+            lds_advect_ = std::max(lds_advect_, la);
+            if (gnext > 0) {
+                BoxGroup &G = groups_[gnext - 1];
+                G.lds_f[fr] = std::max(G.lds_f[fr], lf);
+                G.lds_a[fr] = std::max(G.lds_a[fr], la);
+            }
             for (int t = 0; t < nt; t++) {
                 items[fr].push_back({s, t});
                 cost.push_back((double)d.K[fr]);  // lane-slots per tile ~ K (W pencils x 64 lanes x K)
             }
         }
+        for (size_t g = 0; g < groups_.size(); g++)
+            groups_[g].item_n[fr] =
+                (g + 1 < groups_.size() ? groups_[g + 1].item_off[fr] : (int)items[fr].size()) - groups_[g].item_off[fr];
         const int n = (int)items[fr].size();
         int *pt = part.data() + fr * (kQueues + 1);
         if (policy_ == 0) {
@@ -590,19 +688,27 @@ int Engine::build()
                                                       hipMemcpyHostToDevice, st_);
         if (e != hipSuccess) return hip_fail(e, "engine build items");
     }
-    // CFL items: row ranges of about 8192 cells
+    // CFL items: ranges of 1024 units (4 per thread; a unit is a 16-byte
+    // vector of a row, or a cell where rows are not vector-aligned)
     std::vector<CflItem> cfl;
-    for (int s : order) {
-        const SoaGrid &g = boxes_[s].g;
-        const int rows = g.ny * g.nz, chunk = std::max(1, 8192 / g.nx);
-        for (int r = 0; r < rows; r += chunk) cfl.push_back({s, r, std::min(chunk, rows - r)});
+    {
+        size_t gnext = 0;
+        for (size_t oi = 0; oi < order.size(); oi++) {
+            const int s = order[oi];
+            if (gnext < groups_.size() && (int)oi == groups_[gnext].act_off) groups_[gnext++].cfl_off = (int)cfl.size();
+            const long units = cfl_units<float>(boxes_[s].g);
+            for (long e = 0; e < units; e += 1024) cfl.push_back({s, (int)e, (int)std::min(1024L, units - e)});
+        }
+        for (size_t g = 0; g < groups_.size(); g++)
+            groups_[g].cfl_n = (g + 1 < groups_.size() ? groups_[g + 1].cfl_off : (int)cfl.size()) - groups_[g].cfl_off;
     }
     ncfl_ = (int)cfl.size();
     e = ensure_cap(&d_cfl_, &cap_cfl_, std::max(ncfl_, 1));
     if (e == hipSuccess && ncfl_) e = hipMemcpyAsync(d_cfl_, cfl.data(), ncfl_ * sizeof(CflItem), hipMemcpyHostToDevice, st_);
     if (e == hipSuccess) e = hipMemcpyAsync(d_part_, part.data(), part.size() * 4, hipMemcpyHostToDevice, st_);
-    if (e == hipSuccess && !active_.empty())
-        e = hipMemcpyAsync(d_active_, active_.data(), active_.size() * 4, hipMemcpyHostToDevice, st_);
+    // the active slots in list order (a permutation of active_; grouped under schedule 3)
+    if (e == hipSuccess && !order.empty())
+        e = hipMemcpyAsync(d_active_, order.data(), order.size() * 4, hipMemcpyHostToDevice, st_);
     if (e != hipSuccess) return hip_fail(e, "engine build upload");
     // persistent grid: blocks per CU from the occupancy of the heaviest variant
     int occ = 1 << 30;
@@ -624,6 +730,40 @@ int Engine::build()
 
 int Engine::enqueue_step()
 {
+    if (policy_ == 3) {
+        // fork: each group's chain (CFL, dt, 12 sweeps) on a side stream;
+        // groups beyond the side streams queue behind others; join at the end
+        hipError_t e = hipEventRecord(evfork_, st_);
+        const int ng = (int)groups_.size(), ns = std::min(ng, kAux);
+        for (int i = 0; i < ns && e == hipSuccess; i++) e = hipStreamWaitEvent(aux_[i], evfork_, 0);
+        for (int g = 0; g < ng && e == hipSuccess; g++) {
+            const BoxGroup &G = groups_[g];
+            hipStream_t S = aux_[g % kAux];
+            if (avg_)
+                hipLaunchKernelGGL((batch_cfl_kernel<true>), dim3(G.cfl_n), dim3(256), 0, S, d_boxes_, d_cfl_ + G.cfl_off,
+                                   d_cmax_);
+            else
+                hipLaunchKernelGGL((batch_cfl_kernel<false>), dim3(G.cfl_n), dim3(256), 0, S, d_boxes_,
+                                   d_cfl_ + G.cfl_off, d_cmax_);
+            e = hipGetLastError();
+            if (e == hipSuccess) {
+                hipLaunchKernelGGL(batch_dt_kernel, dim3(1), dim3(256), 0, S, d_active_ + G.act_off, G.act_n, d_cmax_,
+                                   d_dt_, d_t_, d_t64_, d_tf_, d_dts_, cap_, d_iter_, d_ctr_, 0);
+                e = hipGetLastError();
+            }
+            for (int sw = 0; sw < 12 && e == hipSuccess; sw++) {
+                const int fr = kSweepFrameB[sw];
+                const bool adv = kSweepOpB[sw] == 1;
+                e = launch_group(fr, G.K[fr], adv, G.item_n[fr], adv ? G.lds_a[fr] : G.lds_f[fr], S, d_boxes_,
+                                 d_items_[fr] + G.item_off[fr], d_dt_, flip_before(sw));
+            }
+        }
+        for (int i = 0; i < ns && e == hipSuccess; i++) {
+            e = hipEventRecord(evjoin_[i], aux_[i]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(st_, evjoin_[i], 0);
+        }
+        return e == hipSuccess ? MHD_OK : hip_fail(e, "engine step (grouped)");
+    }
     if (avg_)
         hipLaunchKernelGGL((batch_cfl_kernel<true>), dim3(ncfl_), dim3(256), 0, st_, d_boxes_, d_cfl_, d_cmax_);
     else
@@ -679,18 +819,22 @@ int Engine::capture()
     return MHD_OK;
 }
 
-int Engine::step(int nsteps)
+int Engine::prepare()
 {
-    if (nsteps < 0) return fail(MHD_ERR_ARG, "step: nsteps < 0");
-    if (active_.empty() || nsteps == 0) return MHD_OK;
+    if (active_.empty()) return MHD_OK;
     if (dirty_) {
         int rc = build();
         if (rc) return rc;
     }
-    if (!exec_) {
-        int rc = capture();
-        if (rc) return rc;
-    }
+    return exec_ ? MHD_OK : capture();
+}
+
+int Engine::step(int nsteps)
+{
+    if (nsteps < 0) return fail(MHD_ERR_ARG, "step: nsteps < 0");
+    if (active_.empty() || nsteps == 0) return MHD_OK;
+    int rc = prepare();
+    if (rc) return rc;
     for (int i = 0; i < nsteps; i++) {
         hipError_t e = hipGraphLaunch(exec_, st_);
         if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
@@ -786,7 +930,8 @@ int Engine::restore_all(std::vector<float *> &saved)
 // Times candidate schedules on the batch itself (trial_steps each, after one
 // warm-up step), then restores every box's state and clock and keeps the
 // fastest.  Candidates: the persistent sweeps with {1, 2, 3, occupancy}
-// blocks per CU under both queue policies, and one block per work item.
+// blocks per CU under both queue policies, one block per work item, and the
+// K-grouped concurrent chains.
 int Engine::autotune(int trial_steps, int *bpc_out, int *policy_out, float *best_ms)
 {
     if (active_.empty()) return fail(MHD_ERR_STATE, "autotune: no active boxes");
@@ -796,9 +941,9 @@ int Engine::autotune(int trial_steps, int *bpc_out, int *policy_out, float *best
     if (rc) return rc;
     float best = INFINITY;
     int bb = 0, bp = 0;
-    for (int policy = 0; policy < 3 && !rc; policy++)
+    for (int policy = 0; policy < 4 && !rc; policy++)
         for (int bpc : {1, 2, 3, 0}) {
-            if (policy == 2 && bpc != 0) continue;
+            if (policy >= 2 && bpc != 0) continue;
             rc = set_schedule(bpc, policy);
             if (!rc) rc = step(1);
             float ms = 0;

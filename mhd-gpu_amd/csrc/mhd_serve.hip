@@ -63,6 +63,7 @@ struct mhd_server {
     size_t resident = 0;
     // statistics
     double zone_updates = 0, busy_s = 0;
+    double t_upload = 0, t_rebuild = 0, t_step = 0, t_download = 0;  // service-thread seconds
     long long steps_run = 0, rebuilds = 0;
     Clock::time_point first_arrival, last_done;
     bool any = false;
@@ -100,6 +101,7 @@ void mhd_server::loop_batched()
                 admit.push_back(r);
             }
         }
+        const auto tu = Clock::now();
         for (Request *r : admit) {
             int rc = eng->add_box(r->nx, r->ny, r->nz, &r->slot);
             if (!rc) rc = eng->set_state(r->slot, r->in_u, r->in_b);
@@ -119,12 +121,29 @@ void mhd_server::loop_batched()
             active.push_back(r);
             changed = true;
         }
+        t_upload += secs(tu, Clock::now());
         if (active.empty()) continue;
         if (changed) {
             std::vector<int> slots;
             for (Request *r : active) slots.push_back(r->slot);
             eng->set_active(slots);
             rebuilds++;
+        }
+        const auto tb = Clock::now();
+        int prc = eng->prepare();
+        t_rebuild += secs(tb, Clock::now());
+        if (prc) {
+            std::lock_guard<std::mutex> lk(mu);
+            fatal = prc;
+            snprintf(fatal_msg, sizeof fatal_msg, "%s", mhd_last_error());
+            for (Request *r : active) {
+                r->rc = prc;
+                r->done = Clock::now();
+                finished.push_back(r);
+            }
+            active.clear();
+            stop = true;
+            return;
         }
         // steps until the next completion, capped so that arrivals are
         // admitted within ~10 ms of GPU time
@@ -151,7 +170,9 @@ void mhd_server::loop_batched()
         }
         est_step_ms = 0.5 * est_step_ms + 0.5 * (1e3 * dt_s / k);
         busy_s += dt_s;
+        t_step += dt_s;
         steps_run += k;
+        const auto td = Clock::now();
         std::vector<Request *> still;
         for (Request *r : active) {
             zone_updates += (double)r->nx * r->ny * r->nz * k;
@@ -169,6 +190,7 @@ void mhd_server::loop_batched()
             last_done = r->done;
             finished.push_back(r);
         }
+        t_download += secs(td, Clock::now());
         if (still.size() != active.size()) {
             {
                 std::lock_guard<std::mutex> lk(mu);
@@ -334,6 +356,20 @@ int mhd_server_stats(mhd_server *sv, double *out7)
     out7[4] = (double)sv->rebuilds;
     out7[5] = (double)sv->resident;
     out7[6] = (double)sv->budget;
+    return MHD_OK;
+}
+
+// out[4]: service-thread seconds spent uploading admitted requests, rebuilding
+// the work lists and step graph, stepping (launch and wait), and reading
+// finished requests back (mode 0; zeros in mode 1)
+int mhd_server_timing(mhd_server *sv, double *out4)
+{
+    if (!sv || !out4) return fail(MHD_ERR_ARG, "mhd_server_timing: bad argument");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    out4[0] = sv->t_upload;
+    out4[1] = sv->t_rebuild;
+    out4[2] = sv->t_step;
+    out4[3] = sv->t_download;
     return MHD_OK;
 }
 

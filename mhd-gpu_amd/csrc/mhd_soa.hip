@@ -63,30 +63,8 @@ template <typename T, bool AVG>
 __global__ void __launch_bounds__(256) cfl_soa_kernel(const T *__restrict__ u, const T *__restrict__ b, SoaGrid g,
                                                       unsigned long long *cmax)
 {
-    const long N = g.N;
-    T c = (T)0;
-    for (int r = blockIdx.x; r < g.ny * g.nz; r += gridDim.x) {
-        const int j = r % g.ny, k = r / g.ny;
-        const long row = (long)j * g.sy + (long)k * g.sz;
-        for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
-            const long s = row + i;
-            const T b1 = b[s], b2 = b[N + s], b3 = b[2 * N + s];
-            T B;
-            if constexpr (AVG) {
-                const long sip = row + (i + 1 == g.nx ? 0 : i + 1);
-                const long sjp = (long)(j + 1 == g.ny ? 0 : j + 1) * g.sy + (long)k * g.sz + i;
-                const long skp = (long)j * g.sy + (long)(k + 1 == g.nz ? 0 : k + 1) * g.sz + i;
-                const T bxa = (T)((double)(b1 + b[sip]) / 2.0);
-                const T bya = (T)((double)(b2 + b[N + sjp]) / 2.0);
-                const T bza = (T)((double)(b3 + b[2 * N + skp]) / 2.0);
-                B = bxa * bxa + bya * bya + bza * bza;
-            } else {
-                B = b1 * b1 + b2 * b2 + b3 * b3;
-            }
-            const T sp = cfl_speed<T>(u[s], u[N + s], u[2 * N + s], u[3 * N + s], u[4 * N + s], B);
-            c = c > sp ? c : sp;  // max_2num (calcfl.c:55)
-        }
-    }
+    T c = cfl_range<T, AVG>(u, b, g, (long)blockIdx.x * blockDim.x + threadIdx.x, cfl_units<T>(g),
+                            (long)gridDim.x * blockDim.x);
     c = wave_max(c);
     __shared__ T red[4];
     if (lane_id() == 0) red[threadIdx.x >> 6] = c;
@@ -330,10 +308,14 @@ static int row_grid(const SoaGrid &g)
 template <typename T>
 hipError_t launch_cfl_soa(const T *u, const T *b, const SoaGrid &g, int avg, unsigned long long *cmax, hipStream_t st)
 {
+    // about 4 units per thread; beyond 8192 blocks the grid strides
+    const long units = cfl_units<T>(g);
+    const long nb = (units + 1023) / 1024;
+    const int grid = (int)(nb < 8192 ? nb : 8192);
     if (avg)
-        hipLaunchKernelGGL((cfl_soa_kernel<T, true>), dim3(row_grid(g)), dim3(256), 0, st, u, b, g, cmax);
+        hipLaunchKernelGGL((cfl_soa_kernel<T, true>), dim3(grid), dim3(256), 0, st, u, b, g, cmax);
     else
-        hipLaunchKernelGGL((cfl_soa_kernel<T, false>), dim3(row_grid(g)), dim3(256), 0, st, u, b, g, cmax);
+        hipLaunchKernelGGL((cfl_soa_kernel<T, false>), dim3(grid), dim3(256), 0, st, u, b, g, cmax);
     return hipGetLastError();
 }
 

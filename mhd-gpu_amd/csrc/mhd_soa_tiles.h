@@ -35,6 +35,93 @@ __device__ __forceinline__ int wrapn(int j, int nj)
 }
 
 // ---------------------------------------------------------------------------
+// CFL (calcfl.c:13-65) over a range of units of one box.  A unit is one
+// 16-byte vector of a row (Vec16<T>::n cells) where the layout allows it,
+// else one cell; units are numbered row by row (rows j + ny*k), so every
+// thread of a block is busy whatever nx is.  Returns the thread's max of the
+// per-cell speed (max_2num, calcfl.c:55); the speeds are >= 0.
+// ---------------------------------------------------------------------------
+template <typename T> __host__ __device__ __forceinline__ bool cfl_vec(const SoaGrid &g)
+{
+    constexpr int VE = 16 / sizeof(T);
+    return g.nx % VE == 0 && g.sy % VE == 0 && g.sz % VE == 0 && g.N % VE == 0;
+}
+template <typename T> __host__ __device__ __forceinline__ long cfl_units(const SoaGrid &g)
+{
+    return (long)(cfl_vec<T>(g) ? g.nx / (16 / (int)sizeof(T)) : g.nx) * g.ny * g.nz;
+}
+
+template <typename T, bool AVG>
+__device__ __forceinline__ T cfl_speed_at(const T *__restrict__ u, const T *__restrict__ b, const SoaGrid &g, long s,
+                                          T b1, T b2, T b3, long sip, long sjp, long skp)
+{
+    const long N = g.N;
+    T B;
+    if constexpr (AVG) {
+        // cell-averaged field (OpenMP/mhd.f90:112-120)
+        const T bxa = (T)((double)(b1 + b[sip]) / 2.0);
+        const T bya = (T)((double)(b2 + b[N + sjp]) / 2.0);
+        const T bza = (T)((double)(b3 + b[2 * N + skp]) / 2.0);
+        B = bxa * bxa + bya * bya + bza * bza;
+    } else {
+        B = b1 * b1 + b2 * b2 + b3 * b3;  // face values, calcfl.c:39-50
+    }
+    return cfl_speed<T>(u[s], u[N + s], u[2 * N + s], u[3 * N + s], u[4 * N + s], B);
+}
+
+template <typename T, bool AVG>
+__device__ __forceinline__ T cfl_range(const T *__restrict__ u, const T *__restrict__ b, const SoaGrid &g, long e,
+                                       long end, long stride)
+{
+    constexpr int VE = Vec16<T>::n;
+    const bool vec = cfl_vec<T>(g);
+    const unsigned nv = vec ? g.nx / VE : g.nx;
+    const long N = g.N;
+    T c = (T)0;
+    for (; e < end; e += stride) {
+        const unsigned r = (unsigned)e / nv, iu = (unsigned)e - r * nv;
+        const unsigned k = r / (unsigned)g.ny, j = r - k * g.ny;
+        const long row = (long)j * g.sy + (long)k * g.sz;
+        const long rowjp = (long)(j + 1 == (unsigned)g.ny ? 0 : j + 1) * g.sy + (long)k * g.sz;
+        const long rowkp = (long)j * g.sy + (long)(k + 1 == (unsigned)g.nz ? 0 : k + 1) * g.sz;
+        if (vec) {
+            const int i0 = iu * VE;
+            const long s = row + i0;
+            V16<T> U[5], Bv[3];
+#pragma unroll
+            for (int q = 0; q < 5; q++) U[q] = *reinterpret_cast<const V16<T> *>(u + q * N + s);
+#pragma unroll
+            for (int q = 0; q < 3; q++) Bv[q] = *reinterpret_cast<const V16<T> *>(b + q * N + s);
+#pragma unroll
+            for (int q = 0; q < VE; q++) {
+                const T b1 = vget<T>(Bv[0], q), b2 = vget<T>(Bv[1], q), b3 = vget<T>(Bv[2], q);
+                T B;
+                if constexpr (AVG) {
+                    const int ip = i0 + q + 1 == g.nx ? 0 : i0 + q + 1;
+                    const T b1p = q + 1 < VE ? vget<T>(Bv[0], q + 1) : b[row + ip];
+                    const T bxa = (T)((double)(b1 + b1p) / 2.0);
+                    const T bya = (T)((double)(b2 + b[N + rowjp + i0 + q]) / 2.0);
+                    const T bza = (T)((double)(b3 + b[2 * N + rowkp + i0 + q]) / 2.0);
+                    B = bxa * bxa + bya * bya + bza * bza;
+                } else {
+                    B = b1 * b1 + b2 * b2 + b3 * b3;
+                }
+                const T sp = cfl_speed<T>(vget<T>(U[0], q), vget<T>(U[1], q), vget<T>(U[2], q), vget<T>(U[3], q),
+                                          vget<T>(U[4], q), B);
+                c = c > sp ? c : sp;
+            }
+        } else {
+            const int i = iu;
+            const long s = row + i;
+            const T sp = cfl_speed_at<T, AVG>(u, b, g, s, b[s], b[N + s], b[2 * N + s],
+                                              row + (i + 1 == g.nx ? 0 : i + 1), rowjp + i, rowkp + i);
+            c = c > sp ? c : sp;
+        }
+    }
+    return c;
+}
+
+// ---------------------------------------------------------------------------
 // tile geometry
 // ---------------------------------------------------------------------------
 struct TileCtx {

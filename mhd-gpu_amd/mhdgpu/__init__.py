@@ -81,6 +81,7 @@ SIGNATURES = {
     "mhd_server_poll": (_I, [_P, ctypes.POINTER(ctypes.c_longlong), _DP, _DP, _IP, _I, _IP]),
     "mhd_server_pending": (_I, [_P, _IP, _IP]),
     "mhd_server_stats": (_I, [_P, _DP]),
+    "mhd_server_timing": (_I, [_P, _DP]),
     "mhd_server_destroy": (_I, [_P]),
 }
 
@@ -403,7 +404,11 @@ class Server:
         out = np.zeros(7)
         check(lib().mhd_server_stats(self.sv, out.ctypes.data_as(_DP)))
         keys = ["zone_updates", "span_s", "gpu_busy_s", "steps", "rebuilds", "resident_bytes", "budget_bytes"]
-        return dict(zip(keys, out.tolist()))
+        d = dict(zip(keys, out.tolist()))
+        t = np.zeros(4)
+        check(lib().mhd_server_timing(self.sv, t.ctypes.data_as(_DP)))
+        d.update(zip(["t_upload_s", "t_rebuild_s", "t_step_s", "t_download_s"], t.tolist()))
+        return d
 
     def close(self):
         if self.sv:
