@@ -10,6 +10,7 @@
 // steps (the server admits and retires requests that way).  FP32 only, the
 // reference's precision.
 #pragma once
+#include <mutex>
 #include <vector>
 
 #include "mhd_soa.h"
@@ -56,8 +57,12 @@ public:
     int remove_box(int slot);
     int set_state(int slot, const float *h_u, const float *h_b);
     int get_state(int slot, float *h_u, float *h_b);
-    // async variants through a caller-provided pinned host buffer: the
-    // caller must keep h_u/h_b alive until the stream has passed them
+    // the serving path: a slot filled (stage_in) or read back and released
+    // (stage_out) on the engine's transfer stream, without waiting for the
+    // compute stream; the slot must not be in the active set meanwhile.
+    // Safe to call from another thread than the one that steps.
+    int stage_in(int nx, int ny, int nz, const float *h_u, const float *h_b, int *slot);
+    int stage_out(int slot, float *h_u, float *h_b);
     int reset_clock(int slot, double tf);
     int dts(int slot, double *h, int max, int *count);
     int time(int slot, double *t, double *t64);
@@ -81,6 +86,15 @@ public:
     int autotune(int trial_steps, int *bpc, int *policy, float *best_ms);
 
 private:
+    struct Chunk {
+        float *base;  // 11 planes: u (5), b (3), b2 (3)
+        long N;
+    };
+    int alloc_box(long N, float **base);
+    void release_box(float *base, long N);
+    int claim_slot(int nx, int ny, int nz, hipStream_t st, int *slot);
+    int reset_clock_on(int slot, double tf, hipStream_t st);
+    int transfer(int slot, void *h_u, void *h_b, int to_device, hipStream_t st);
     int build();
     int capture();
     void drop_graph();
@@ -89,9 +103,14 @@ private:
     int restore_all(std::vector<float *> &saved);
 
     int max_boxes_, cap_, avg_;
-    hipStream_t st_ = nullptr;
+    hipStream_t st_ = nullptr;   // compute (steps)
+    hipStream_t xs_ = nullptr;   // transfers of the serving path
+    std::mutex mu_;              // slots and the memory pool
+    std::vector<Chunk> pool_;    // released box memory, reused by size
+    float *xbuf_[2] = {nullptr, nullptr};  // AoS transfer buffers of st_ and xs_
+    size_t xcap_[2] = {0, 0};
     std::vector<BoxDesc> boxes_;   // host copy, per slot
-    std::vector<bool> used_;
+    std::vector<char> used_;  // bytes, not bits: two threads touch neighbouring slots
     std::vector<int> active_;
     std::vector<double> tf_;
     // device arrays

@@ -328,12 +328,12 @@ Engine::~Engine()
 {
     if (st_) (void)hipStreamSynchronize(st_);
     drop_graph();
+    if (xs_) (void)hipStreamSynchronize(xs_);
     for (size_t s = 0; s < boxes_.size(); s++)
-        if (used_[s]) {
-            (void)hipFree(boxes_[s].u);
-            (void)hipFree(boxes_[s].b);
-            (void)hipFree(boxes_[s].b2);
-        }
+        if (used_[s]) (void)hipFree(boxes_[s].u);
+    for (const Chunk &c : pool_) (void)hipFree(c.base);
+    for (float *x : xbuf_)
+        if (x) (void)hipFree(x);
     void *ps[] = {d_boxes_, d_dt_, d_t_, d_tf_, d_dts_, d_t64_, d_cmax_, d_iter_, d_active_, d_ctr_, d_part_,
                   d_items_[0], d_items_[1], d_items_[2], d_cfl_};
     for (void *p : ps)
@@ -349,12 +349,14 @@ Engine::~Engine()
         }
     }
     if (st_) (void)hipStreamDestroy(st_);
+    if (xs_) (void)hipStreamDestroy(xs_);
 }
 
 int Engine::init()
 {
     if (max_boxes_ < 1 || cap_ < 1) return fail(MHD_ERR_ARG, "engine: bad capacity");
     hipError_t e = hipStreamCreate(&st_);
+    if (e == hipSuccess) e = hipStreamCreate(&xs_);
     if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
     const size_t B = max_boxes_;
     struct A {
@@ -419,70 +421,142 @@ long Engine::cells(int slot) const
     return (long)g.nx * g.ny * g.nz;
 }
 
-int Engine::add_box(int nx, int ny, int nz, int *slot)
+// Box memory: one allocation of 11 planes (u, b, b2).  Released boxes go to
+// a pool and are reused for boxes of N <= chunk <= 2N cells per plane, so a
+// serving loop turning requests over calls no hipFree (which waits for the
+// whole device).  On an allocation failure the pool is emptied and the
+// allocation retried.
+int Engine::alloc_box(long N, float **base)
+{
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        int best = -1;
+        for (size_t i = 0; i < pool_.size(); i++)
+            if (pool_[i].N >= N && pool_[i].N <= 2 * N && (best < 0 || pool_[i].N < pool_[best].N)) best = (int)i;
+        if (best >= 0) {
+            *base = pool_[best].base;
+            pool_.erase(pool_.begin() + best);
+            return MHD_OK;
+        }
+    }
+    const size_t bytes = 11 * (size_t)N * 4;
+    hipError_t e = hipMalloc((void **)base, bytes);
+    if (e != hipSuccess) {
+        std::vector<Chunk> drop;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            drop.swap(pool_);
+        }
+        for (const Chunk &c : drop) (void)hipFree(c.base);
+        e = hipMalloc((void **)base, bytes);
+    }
+    return e == hipSuccess ? MHD_OK : fail(MHD_ERR_NOMEM, "engine: box of %ld cells: %s", N, hipGetErrorString(e));
+}
+
+void Engine::release_box(float *base, long N)
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    pool_.push_back({base, N});
+    if (pool_.size() > 32) {  // keep the pool small: the oldest chunk goes
+        (void)hipFree(pool_.front().base);
+        pool_.erase(pool_.begin());
+    }
+}
+
+int Engine::claim_slot(int nx, int ny, int nz, hipStream_t st, int *slot)
 {
     int rc = check_shape3(nx, ny, nz);
     if (rc) return rc;
-    int s = -1;
-    for (int i = 0; i < max_boxes_; i++)
-        if (!used_[i]) {
-            s = i;
-            break;
-        }
-    if (s < 0) return fail(MHD_ERR_FULL, "engine: all %d slots in use", max_boxes_);
     BoxDesc d;
     memset(&d, 0, sizeof d);
     d.g = make_grid(nx, ny, nz, soa_pad_y(4), soa_pad_z(4));
-    const size_t N = (size_t)d.g.N;
-    hipError_t e = hipMalloc((void **)&d.u, 5 * N * 4);
-    if (e == hipSuccess) e = hipMalloc((void **)&d.b, 3 * N * 4);
-    if (e == hipSuccess) e = hipMalloc((void **)&d.b2, 3 * N * 4);
-    if (e != hipSuccess) {
-        if (d.u) (void)hipFree(d.u);
-        if (d.b) (void)hipFree(d.b);
-        if (d.b2) (void)hipFree(d.b2);
-        return fail(MHD_ERR_NOMEM, "engine: box %dx%dx%d: %s", nx, ny, nz, hipGetErrorString(e));
-    }
+    const long N = d.g.N;
+    float *base = nullptr;
+    rc = alloc_box(N, &base);
+    if (rc) return rc;
+    d.u = base;
+    d.b = base + 5 * N;
+    d.b2 = base + 8 * N;
     for (int fr = 0; fr < 3; fr++) {
         d.fr[fr] = make_soa_box<float>(fr, d.g);
         d.K[fr] = pick_k(d.fr[fr].f.ni);
         d.ntj[fr] = (d.fr[fr].f.nj + kBW - 1) / kBW;
     }
-    boxes_[s] = d;
-    used_[s] = true;
-    e = hipMemcpyAsync(d_boxes_ + s, &boxes_[s], sizeof(BoxDesc), hipMemcpyHostToDevice, st_);
-    if (e == hipSuccess) e = hipStreamSynchronize(st_);
-    if (e != hipSuccess) return hip_fail(e, "engine add_box");
-    rc = reset_clock(s, INFINITY);
-    if (rc) return rc;
+    int s = -1;
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (int i = 0; i < max_boxes_; i++)
+            if (!used_[i]) {
+                s = i;
+                break;
+            }
+        if (s >= 0) {
+            boxes_[s] = d;
+            used_[s] = true;
+        }
+    }
+    if (s < 0) {
+        release_box(base, N);
+        return fail(MHD_ERR_FULL, "engine: all %d slots in use", max_boxes_);
+    }
+    hipError_t e = hipMemcpyAsync(d_boxes_ + s, &boxes_[s], sizeof(BoxDesc), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess) rc = reset_clock_on(s, INFINITY, st);
+    else rc = hip_fail(e, "engine add_box");
+    if (rc) {
+        remove_box(s);
+        return rc;
+    }
     *slot = s;
     return MHD_OK;
+}
+
+int Engine::add_box(int nx, int ny, int nz, int *slot)
+{
+    hipError_t e = hipStreamSynchronize(st_);
+    if (e != hipSuccess) return hip_fail(e, "engine add_box");
+    return claim_slot(nx, ny, nz, st_, slot);
 }
 
 int Engine::remove_box(int slot)
 {
     if (slot < 0 || slot >= max_boxes_ || !used_[slot]) return fail(MHD_ERR_ARG, "engine: bad slot %d", slot);
     hipError_t e = hipStreamSynchronize(st_);
+    if (e == hipSuccess) e = hipStreamSynchronize(xs_);
     if (e != hipSuccess) return hip_fail(e, "engine remove_box");
     auto it = std::find(active_.begin(), active_.end(), slot);
     if (it != active_.end()) {
         active_.erase(it);
         dirty_ = true;
     }
-    (void)hipFree(boxes_[slot].u);
-    (void)hipFree(boxes_[slot].b);
-    (void)hipFree(boxes_[slot].b2);
-    memset(&boxes_[slot], 0, sizeof(BoxDesc));
-    used_[slot] = false;
+    BoxDesc d;
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        d = boxes_[slot];
+        memset(&boxes_[slot], 0, sizeof(BoxDesc));
+        used_[slot] = false;
+    }
+    release_box(d.u, d.g.N);
     return MHD_OK;
 }
 
-static int transfer(BoxDesc &d, void *h_u, void *h_b, int to_device, hipStream_t st)
+// AoS host arrays <-> the slot's SoA planes through the stream's AoS buffer
+int Engine::transfer(int slot, void *h_u, void *h_b, int to_device, hipStream_t st)
 {
+    BoxDesc &d = boxes_[slot];
     const long nc = (long)d.g.nx * d.g.ny * d.g.nz;
-    void *tmp = nullptr;
-    hipError_t e = hipMalloc(&tmp, 5 * nc * 4);
-    if (e != hipSuccess) return fail(MHD_ERR_NOMEM, "state transfer buffer: %s", hipGetErrorString(e));
+    const int xi = st == xs_ ? 1 : 0;
+    const size_t need = 5 * (size_t)nc * 4;
+    hipError_t e = hipSuccess;
+    if (xcap_[xi] < need) {
+        if (xbuf_[xi]) (void)hipFree(xbuf_[xi]);
+        xbuf_[xi] = nullptr;
+        xcap_[xi] = 0;
+        e = hipMalloc((void **)&xbuf_[xi], need);
+        if (e != hipSuccess) return fail(MHD_ERR_NOMEM, "state transfer buffer: %s", hipGetErrorString(e));
+        xcap_[xi] = need;
+    }
+    float *tmp = xbuf_[xi];
     struct P {
         void *h;
         float *d;
@@ -492,14 +566,13 @@ static int transfer(BoxDesc &d, void *h_u, void *h_b, int to_device, hipStream_t
         if (e != hipSuccess) break;
         if (to_device) {
             e = hipMemcpyAsync(tmp, p.h, (size_t)nc * p.nc * 4, hipMemcpyHostToDevice, st);
-            if (e == hipSuccess) e = launch_aos_soa<float>(p.d, (float *)tmp, p.nc, d.g, 1, st);
+            if (e == hipSuccess) e = launch_aos_soa<float>(p.d, tmp, p.nc, d.g, 1, st);
         } else {
-            e = launch_aos_soa<float>(p.d, (float *)tmp, p.nc, d.g, 0, st);
+            e = launch_aos_soa<float>(p.d, tmp, p.nc, d.g, 0, st);
             if (e == hipSuccess) e = hipMemcpyAsync(p.h, tmp, (size_t)nc * p.nc * 4, hipMemcpyDeviceToHost, st);
         }
         if (e == hipSuccess) e = hipStreamSynchronize(st);
     }
-    (void)hipFree(tmp);
     return e == hipSuccess ? MHD_OK : hip_fail(e, "state transfer");
 }
 
@@ -508,27 +581,71 @@ int Engine::set_state(int slot, const float *h_u, const float *h_b)
     if (slot < 0 || slot >= max_boxes_ || !used_[slot] || !h_u || !h_b) return fail(MHD_ERR_ARG, "set_state");
     hipError_t e = hipStreamSynchronize(st_);
     if (e != hipSuccess) return hip_fail(e, "set_state");
-    return transfer(boxes_[slot], const_cast<float *>(h_u), const_cast<float *>(h_b), 1, st_);
+    return transfer(slot, const_cast<float *>(h_u), const_cast<float *>(h_b), 1, st_);
 }
 
 int Engine::get_state(int slot, float *h_u, float *h_b)
 {
     if (slot < 0 || slot >= max_boxes_ || !used_[slot] || !h_u || !h_b) return fail(MHD_ERR_ARG, "get_state");
-    return transfer(boxes_[slot], h_u, h_b, 0, st_);
+    return transfer(slot, h_u, h_b, 0, st_);
+}
+
+int Engine::stage_in(int nx, int ny, int nz, const float *h_u, const float *h_b, int *slot)
+{
+    if (!h_u || !h_b || !slot) return fail(MHD_ERR_ARG, "stage_in: bad argument");
+    int s = -1;
+    int rc = claim_slot(nx, ny, nz, xs_, &s);
+    if (rc) return rc;
+    rc = transfer(s, const_cast<float *>(h_u), const_cast<float *>(h_b), 1, xs_);
+    if (rc) {
+        BoxDesc d;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            d = boxes_[s];
+            memset(&boxes_[s], 0, sizeof(BoxDesc));
+            used_[s] = false;
+        }
+        release_box(d.u, d.g.N);
+        return rc;
+    }
+    *slot = s;
+    return MHD_OK;
+}
+
+int Engine::stage_out(int slot, float *h_u, float *h_b)
+{
+    if (slot < 0 || slot >= max_boxes_ || !used_[slot]) return fail(MHD_ERR_ARG, "stage_out: bad slot %d", slot);
+    const int rc = (h_u && h_b) ? transfer(slot, h_u, h_b, 0, xs_) : MHD_OK;
+    BoxDesc d;
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        d = boxes_[slot];
+        memset(&boxes_[slot], 0, sizeof(BoxDesc));
+        used_[slot] = false;
+    }
+    release_box(d.u, d.g.N);
+    return rc;
 }
 
 int Engine::reset_clock(int slot, double tf)
 {
     if (slot < 0 || slot >= max_boxes_ || !used_[slot]) return fail(MHD_ERR_ARG, "reset_clock: bad slot");
+    return reset_clock_on(slot, tf, st_);
+}
+
+// the slot's clock entries: tf, t = dt = t64 = 0, no steps, CFL max 0 (small
+// fills on the given stream, outside any captured graph)
+int Engine::reset_clock_on(int slot, double tf, hipStream_t st)
+{
     tf_[slot] = tf;
     const float tff = (float)tf;
-    hipError_t e = hipMemcpyAsync(d_tf_ + slot, &tff, 4, hipMemcpyHostToDevice, st_);
-    if (e == hipSuccess) e = hipMemsetAsync(d_t_ + slot, 0, 4, st_);
-    if (e == hipSuccess) e = hipMemsetAsync(d_dt_ + slot, 0, 4, st_);
-    if (e == hipSuccess) e = hipMemsetAsync(d_t64_ + slot, 0, 8, st_);
-    if (e == hipSuccess) e = hipMemsetAsync(d_iter_ + slot, 0, 4, st_);
-    if (e == hipSuccess) e = hipMemsetAsync(d_cmax_ + slot, 0, 8, st_);
-    if (e == hipSuccess) e = hipStreamSynchronize(st_);
+    hipError_t e = hipMemcpyAsync(d_tf_ + slot, &tff, 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemsetAsync(d_t_ + slot, 0, 4, st);
+    if (e == hipSuccess) e = hipMemsetAsync(d_dt_ + slot, 0, 4, st);
+    if (e == hipSuccess) e = hipMemsetAsync(d_t64_ + slot, 0, 8, st);
+    if (e == hipSuccess) e = hipMemsetAsync(d_iter_ + slot, 0, 4, st);
+    if (e == hipSuccess) e = hipMemsetAsync(d_cmax_ + slot, 0, 8, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     return e == hipSuccess ? MHD_OK : hip_fail(e, "reset_clock");
 }
 

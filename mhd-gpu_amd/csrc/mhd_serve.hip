@@ -2,9 +2,12 @@
 // are admitted in arrival order while their device state fits an HBM budget,
 // stay resident on the device, and advance together through the batched
 // engine (continuous batching: a request that arrives while others run joins
-// at the next step boundary, a finished one leaves).  A service thread owns
-// the GPU work; submit/poll are thread-safe.  Mode 1 is the FIFO baseline:
-// one request at a time through the plain device-resident path.
+// at the next step boundary, a finished one leaves).  Two threads own the GPU
+// work: the service thread steps the active set on the engine's compute
+// stream; the staging thread uploads admitted requests and reads finished
+// ones back on the engine's transfer stream, so those copies overlap the
+// steps of the others.  submit/poll are thread-safe.  Mode 1 is the FIFO
+// baseline: one request at a time through the plain device-resident path.
 #include <math.h>
 #include <string.h>
 
@@ -56,8 +59,14 @@ struct mhd_server {
     std::thread th;
     std::mutex mu;
     std::condition_variable cv;
-    std::deque<Request *> queue;
+    std::deque<Request *> queue;     // submitted, not admitted
+    std::deque<Request *> ready;     // uploaded, joining at the next chunk
+    std::deque<Request *> outgoing;  // stepped to the end, to be read back
     std::vector<Request *> active, finished;
+    int staging = 0;                 // being uploaded
+    int in_flight = 0;               // admitted and not finished
+    bool service_done = false;
+    std::thread th_stage;
     long long next_id = 1;
     bool stop = false;
     size_t resident = 0;
@@ -70,59 +79,136 @@ struct mhd_server {
     int fatal = MHD_OK;
     char fatal_msg[256] = {0};
 
+    bool admissible() const
+    {
+        if (queue.empty() || fatal) return false;
+        const Request *r = queue.front();
+        return in_flight < max_active && resident + r->bytes <= budget &&
+               (active_cells == 0 || active_cells + r->cells() <= max_cells);
+    }
+    void fail_all(std::deque<Request *> &q, int rc);
     void loop_batched();
+    void loop_stage();
     void loop_fifo();
 };
 
 static double secs(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double>(b - a).count(); }
 
+// (under mu) finish every request of q with rc; admitted ones (holding a
+// slot) give back their slot and their share of the budget
+void mhd_server::fail_all(std::deque<Request *> &q, int rc)
+{
+    for (Request *r : q) {
+        if (r->slot >= 0) {
+            eng->stage_out(r->slot, nullptr, nullptr);
+            r->slot = -1;
+            resident -= r->bytes;
+            active_cells -= r->cells();
+            in_flight--;
+        }
+        r->rc = rc;
+        r->done = Clock::now();
+        finished.push_back(r);
+    }
+    q.clear();
+}
+
+// staging thread: read finished requests back first (their latency is
+// running), else upload the next admissible request
+void mhd_server::loop_stage()
+{
+    (void)hipSetDevice(device);
+    for (;;) {
+        Request *out = nullptr, *in = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] {
+                return !outgoing.empty() || admissible() || fatal ||
+                       (service_done && queue.empty() && ready.empty() && outgoing.empty());
+            });
+            if (!outgoing.empty()) {
+                out = outgoing.front();
+                outgoing.pop_front();
+            } else if (fatal) {
+                // the service thread stopped on an error: nothing more will step
+                fail_all(ready, fatal);
+                fail_all(queue, fatal);
+                if (service_done) return;
+                cv.wait(lk, [&] { return service_done || !outgoing.empty(); });
+                continue;
+            } else if (admissible()) {
+                in = queue.front();
+                queue.pop_front();
+                resident += in->bytes;
+                active_cells += in->cells();
+                in_flight++;
+                staging++;
+            } else {
+                return;  // service done and nothing left
+            }
+        }
+        const auto t0 = Clock::now();
+        if (out) {
+            const int rc = eng->stage_out(out->slot, out->out_u, out->out_b);
+            std::lock_guard<std::mutex> lk(mu);
+            out->rc = rc;
+            out->slot = -1;
+            out->done = Clock::now();
+            resident -= out->bytes;
+            active_cells -= out->cells();
+            in_flight--;
+            last_done = out->done;
+            finished.push_back(out);
+            t_download += secs(t0, out->done);
+        } else {
+            in->admitted = t0;
+            int slot = -1;
+            const int rc = eng->stage_in(in->nx, in->ny, in->nz, in->in_u, in->in_b, &slot);
+            std::lock_guard<std::mutex> lk(mu);
+            staging--;
+            t_upload += secs(t0, Clock::now());
+            if (rc) {
+                in->rc = rc;
+                in->done = Clock::now();
+                resident -= in->bytes;
+                active_cells -= in->cells();
+                in_flight--;
+                finished.push_back(in);
+            } else {
+                in->slot = slot;
+                in->remaining = in->nsteps;
+                ready.push_back(in);
+            }
+        }
+        cv.notify_all();
+    }
+}
+
+// service thread: join the uploaded requests, step the active set in chunks,
+// hand the finished ones to the staging thread
 void mhd_server::loop_batched()
 {
     (void)hipSetDevice(device);
     double est_step_ms = 1.0;  // running estimate of one batched step
     for (;;) {
         bool changed = false;
-        std::vector<Request *> admit;
         {
             std::unique_lock<std::mutex> lk(mu);
-            cv.wait(lk, [&] { return stop || !queue.empty() || !active.empty(); });
-            if (stop && queue.empty() && active.empty()) return;
-            // admission in arrival order while the budget and the slots allow
-            // (and, past the first, while the admitted cells stay under the
-            // saturation cap: requests then run in arrival-ordered waves that
-            // fill the GPU instead of all sharing it)
-            while (!queue.empty() && (int)(active.size() + admit.size()) < max_active &&
-                   resident + queue.front()->bytes <= budget &&
-                   (active_cells == 0 || active_cells + queue.front()->cells() <= max_cells)) {
-                Request *r = queue.front();
-                queue.pop_front();
-                resident += r->bytes;
-                active_cells += r->cells();
-                admit.push_back(r);
+            cv.wait(lk, [&] {
+                return !ready.empty() || !active.empty() ||
+                       (stop && queue.empty() && staging == 0 && ready.empty());
+            });
+            if (active.empty() && ready.empty()) {
+                service_done = true;
+                cv.notify_all();
+                return;
+            }
+            while (!ready.empty()) {
+                active.push_back(ready.front());
+                ready.pop_front();
+                changed = true;
             }
         }
-        const auto tu = Clock::now();
-        for (Request *r : admit) {
-            int rc = eng->add_box(r->nx, r->ny, r->nz, &r->slot);
-            if (!rc) rc = eng->set_state(r->slot, r->in_u, r->in_b);
-            r->admitted = Clock::now();
-            r->remaining = r->nsteps;
-            if (rc) {
-                r->rc = rc;
-                r->done = Clock::now();
-                if (r->slot >= 0) eng->remove_box(r->slot);
-                std::lock_guard<std::mutex> lk(mu);
-                resident -= r->bytes;
-                active_cells -= r->cells();
-                finished.push_back(r);
-                continue;
-            }
-            std::lock_guard<std::mutex> lk(mu);
-            active.push_back(r);
-            changed = true;
-        }
-        t_upload += secs(tu, Clock::now());
-        if (active.empty()) continue;
         if (changed) {
             std::vector<int> slots;
             for (Request *r : active) slots.push_back(r->slot);
@@ -130,29 +216,16 @@ void mhd_server::loop_batched()
             rebuilds++;
         }
         const auto tb = Clock::now();
-        int prc = eng->prepare();
+        int rc = eng->prepare();
         t_rebuild += secs(tb, Clock::now());
-        if (prc) {
-            std::lock_guard<std::mutex> lk(mu);
-            fatal = prc;
-            snprintf(fatal_msg, sizeof fatal_msg, "%s", mhd_last_error());
-            for (Request *r : active) {
-                r->rc = prc;
-                r->done = Clock::now();
-                finished.push_back(r);
-            }
-            active.clear();
-            stop = true;
-            return;
-        }
-        // steps until the next completion, capped so that arrivals are
-        // admitted within ~10 ms of GPU time
+        // steps until the next completion, capped so that arrivals join
+        // within ~10 ms of GPU time
         int k = 1 << 30;
         for (Request *r : active) k = std::min(k, r->remaining);
         const int cap = std::max(1, (int)(10.0 / std::max(est_step_ms, 1e-3)));
         k = std::max(1, std::min(k, cap));
         const auto t0 = Clock::now();
-        int rc = eng->step(k);
+        if (!rc) rc = eng->step(k);
         if (!rc) rc = eng->sync();
         const double dt_s = secs(t0, Clock::now());
         if (rc) {
@@ -162,46 +235,40 @@ void mhd_server::loop_batched()
             for (Request *r : active) {
                 r->rc = rc;
                 r->done = Clock::now();
+                resident -= r->bytes;
+                active_cells -= r->cells();
+                in_flight--;
                 finished.push_back(r);
             }
             active.clear();
             stop = true;
+            service_done = true;
+            cv.notify_all();
             return;
         }
         est_step_ms = 0.5 * est_step_ms + 0.5 * (1e3 * dt_s / k);
-        busy_s += dt_s;
-        t_step += dt_s;
-        steps_run += k;
-        const auto td = Clock::now();
-        std::vector<Request *> still;
+        std::vector<Request *> still, done;
         for (Request *r : active) {
-            zone_updates += (double)r->nx * r->ny * r->nz * k;
             r->remaining -= k;
-            if (r->remaining > 0) {
-                still.push_back(r);
-                continue;
-            }
-            r->rc = eng->get_state(r->slot, r->out_u, r->out_b);
-            eng->remove_box(r->slot);
-            r->done = Clock::now();
-            std::lock_guard<std::mutex> lk(mu);
-            resident -= r->bytes;
-            active_cells -= r->cells();
-            last_done = r->done;
-            finished.push_back(r);
+            (r->remaining > 0 ? still : done).push_back(r);
         }
-        t_download += secs(td, Clock::now());
-        if (still.size() != active.size()) {
-            {
-                std::lock_guard<std::mutex> lk(mu);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            busy_s += dt_s;
+            t_step += dt_s;
+            steps_run += k;
+            for (Request *r : active) zone_updates += (double)r->nx * r->ny * r->nz * k;
+            if (!done.empty()) {
+                for (Request *r : done) outgoing.push_back(r);
                 active.swap(still);
             }
-            if (!active.empty()) {
-                std::vector<int> slots;
-                for (Request *r : active) slots.push_back(r->slot);
-                eng->set_active(slots);
-                rebuilds++;
-            }
+        }
+        if (!done.empty()) {
+            cv.notify_all();
+            std::vector<int> slots;
+            for (Request *r : active) slots.push_back(r->slot);
+            eng->set_active(slots);
+            rebuilds++;
         }
     }
 }
@@ -267,6 +334,7 @@ int mhd_server_create(mhd_server **out, int mode, size_t hbm_budget, int max_act
             return rc;
         }
         sv->th = std::thread([sv] { sv->loop_batched(); });
+        sv->th_stage = std::thread([sv] { sv->loop_stage(); });
     } else {
         sv->th = std::thread([sv] { sv->loop_fifo(); });
     }
@@ -337,8 +405,8 @@ int mhd_server_pending(mhd_server *sv, int *queued, int *active)
 {
     if (!sv) return fail(MHD_ERR_ARG, "mhd_server_pending: null server");
     std::lock_guard<std::mutex> lk(sv->mu);
-    if (queued) *queued = (int)sv->queue.size();
-    if (active) *active = (int)sv->active.size();
+    if (queued) *queued = (int)sv->queue.size() + sv->staging + (int)sv->ready.size();
+    if (active) *active = (int)sv->active.size() + (int)sv->outgoing.size();
     return sv->fatal ? fail(sv->fatal, "server stopped: %s", sv->fatal_msg) : MHD_OK;
 }
 
@@ -382,8 +450,11 @@ int mhd_server_destroy(mhd_server *sv)
     }
     sv->cv.notify_all();
     if (sv->th.joinable()) sv->th.join();
+    if (sv->th_stage.joinable()) sv->th_stage.join();
     for (Request *r : sv->finished) delete r;
     for (Request *r : sv->queue) delete r;
+    for (Request *r : sv->ready) delete r;
+    for (Request *r : sv->outgoing) delete r;
     for (Request *r : sv->active) delete r;
     delete sv->eng;
     delete sv;
