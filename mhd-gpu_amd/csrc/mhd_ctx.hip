@@ -27,6 +27,7 @@ struct mhd_ctx {
     hipStream_t st;
     void *u, *b;               // state: 5 + 3 planes of ncell values (SoA, ctx precision)
     void *b2;                  // second field buffer: advectbyzx reads one, writes the other
+    void *fa;                  // scratch plane: the EMF flux of the y/z advectbyzx (pass A -> pass B)
     unsigned long long *cmax;  // CFL max accumulator
     void *dt, *t;              // device scalars in ctx precision
     double *t64;
@@ -91,7 +92,9 @@ template <typename T> static hipError_t enqueue_sweep(mhd_ctx *c, int s)
     T *bnext = (T *)(field_buffer_before(s) ? c->b : c->b2);
     if (kSweepOp[s] == 0)
         return launch_fluidx_soa<T>((T *)c->u, bcur, fr, c->g, (const T *)c->dt, c->st);
-    return launch_advect_soa<T>((const T *)c->u, bcur, bnext, fr, c->g, (const T *)c->dt, c->st);
+    // x frame: LDS tiles (pencils contiguous); y/z frames: column streaming
+    if (fr == 0) return launch_advect_soa<T>((const T *)c->u, bcur, bnext, fr, c->g, (const T *)c->dt, c->st);
+    return launch_advect_stream<T>((const T *)c->u, bcur, bnext, (T *)c->fa, fr, c->g, (const T *)c->dt, c->st);
 }
 
 template <typename T> static hipError_t enqueue_step(mhd_ctx *c)
@@ -171,7 +174,7 @@ int mhd_ctx_create(mhd_ctx **out, int nx, int ny, int nz, int precision, int avg
     struct A {
         void **p;
         size_t n;
-    } al[] = {{&c->u, 5 * NP},   {&c->b, 3 * NP}, {&c->b2, 3 * NP},
+    } al[] = {{&c->u, 5 * NP},   {&c->b, 3 * NP}, {&c->b2, 3 * NP}, {&c->fa, NP},
               {(void **)&c->cmax, 64},
               {&c->dt, 64},                 {&c->t, 64},               {(void **)&c->t64, 64},
               {&c->dts, (size_t)kDtCap * P}, {(void **)&c->iter, 64},  {(void **)&c->mon_part, 6 * 8 * 1024},
@@ -204,7 +207,7 @@ int mhd_ctx_destroy(mhd_ctx *c)
     if (!c) return MHD_OK;
     if (c->st) (void)hipStreamSynchronize(c->st);
     drop_graph(c);
-    void *ps[] = {c->u, c->b, c->b2, c->cmax, c->dt, c->t, c->t64, c->dts, c->iter, c->mon_part, c->mon_out};
+    void *ps[] = {c->u, c->b, c->b2, c->fa, c->cmax, c->dt, c->t, c->t64, c->dts, c->iter, c->mon_part, c->mon_out};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -39,6 +39,11 @@ hipError_t launch_fluidx_soa(T *u, const T *b, int fr, const SoaGrid &g, const T
 template <typename T>
 hipError_t launch_advect_soa(const T *u, const T *bin, T *bout, int fr, const SoaGrid &g, const T *dt,
                              hipStream_t st);
+// advectbyzx of the y (fr 1) or z (fr 2) frame as two column-streaming
+// passes; fa is a scratch plane of N values
+template <typename T>
+hipError_t launch_advect_stream(const T *u, const T *bin, T *bout, T *fa, int fr, const SoaGrid &g, const T *dt,
+                                hipStream_t st);
 template <typename T> hipError_t prepare_soa(int nx, int ny, int nz);
 template <typename T>
 hipError_t launch_cfl_soa(const T *u, const T *b, const SoaGrid &g, int avg, unsigned long long *cmax, hipStream_t st);

@@ -11,6 +11,9 @@
 // arithmetic is the shared core (mhd_core.h), unchanged.
 #include <stdlib.h>
 
+#include <algorithm>
+
+#include "mhd_soa_stream.h"
 #include "mhd_soa_tiles.h"
 
 namespace mhd {
@@ -46,6 +49,26 @@ __global__ void __launch_bounds__(64 * W)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     advect_soa_tile<T, K, W, FR>(u, bin, bout, bx, dtp, xcd_tile(blockIdx.x, gridDim.x), ntj,
                                  reinterpret_cast<T *>(smem_raw));
+}
+
+// column-streaming advectbyzx of the y and z frames (mhd_soa_stream.h):
+// four independent waves per block, blocks dealt XCD-contiguously
+template <typename T, int FR>
+__global__ void __launch_bounds__(256) advect_stream_a_kernel(const T *__restrict__ u, const T *__restrict__ bin,
+                                                              T *__restrict__ bout, T *__restrict__ fa, StreamBox sb,
+                                                              const T *__restrict__ dtp, int nwaves)
+{
+    const int wave = xcd_tile(blockIdx.x, gridDim.x) * 4 + (int)(threadIdx.x >> 6);
+    if (wave < nwaves) advect_stream_a<T, FR>(u, bin, bout, fa, sb, dtp, wave);
+}
+
+template <typename T, int FR>
+__global__ void __launch_bounds__(256) advect_stream_b_kernel(const T *__restrict__ u, const T *__restrict__ bin,
+                                                              T *__restrict__ bout, const T *__restrict__ fa,
+                                                              StreamBox sb, const T *__restrict__ dtp, int nwaves)
+{
+    const int wave = xcd_tile(blockIdx.x, gridDim.x) * 4 + (int)(threadIdx.x >> 6);
+    if (wave < nwaves) advect_stream_b<T, FR>(u, bin, bout, fa, sb, dtp, wave);
 }
 
 // ---------------------------------------------------------------------------
@@ -270,6 +293,58 @@ hipError_t launch_advect_soa(const T *u, const T *bin, T *bout, int fr, const So
 #undef SA2
 }
 
+// Rows per segment: enough segments for about 8192 waves of pass A (8 per
+// SIMD: the streams are latency-bound), but at least 32 rows each (a segment
+// re-reads 8 rows); MHD_STREAM_SEG (rows) overrides.
+StreamBox make_stream_box(int fr, const SoaGrid &g)
+{
+    StreamBox sb;
+    sb.N = g.N;
+    sb.nj = g.nx;
+    if (fr == 1) {
+        sb.ni = g.ny; sb.si = g.sy; sb.nk = g.nz; sb.sk = g.sz;
+    } else {
+        sb.ni = g.nz; sb.si = g.sz; sb.nk = g.ny; sb.sk = g.sy;
+    }
+    sb.ntj = (sb.nj + 63) / 64;
+    static const int env_seg = [] {
+        const char *v = getenv("MHD_STREAM_SEG");
+        return v ? atoi(v) : 0;
+    }();
+    int S;
+    if (env_seg > 0) {
+        S = env_seg;
+    } else {
+        const long pencils = (long)sb.ntj * sb.nk;
+        int nseg = (int)((8192 + pencils - 1) / pencils);
+        nseg = std::max(1, std::min(nseg, sb.ni / 32));
+        S = (sb.ni + nseg - 1) / nseg;
+    }
+    S = std::max(1, std::min(S, sb.ni));
+    sb.S = S;
+    sb.nseg = (sb.ni + S - 1) / S;
+    return sb;
+}
+
+template <typename T>
+hipError_t launch_advect_stream(const T *u, const T *bin, T *bout, T *fa, int fr, const SoaGrid &g, const T *dt,
+                                hipStream_t st)
+{
+    if (fr != 1 && fr != 2) return hipErrorInvalidValue;
+    const StreamBox sb = make_stream_box(fr, g);
+    const int nwa = sb.ntj * sb.nk * sb.nseg;              // pass A: one pencil per lane
+    const int nwb = sb.ntj * ((sb.nk + 1) / 2) * sb.nseg;  // pass B: two planes per lane
+    const dim3 ga((nwa + 3) / 4), gb((nwb + 3) / 4), block(256);
+    if (fr == 1) {
+        hipLaunchKernelGGL((advect_stream_a_kernel<T, 1>), ga, block, 0, st, u, bin, bout, fa, sb, dt, nwa);
+        hipLaunchKernelGGL((advect_stream_b_kernel<T, 1>), gb, block, 0, st, u, bin, bout, fa, sb, dt, nwb);
+    } else {
+        hipLaunchKernelGGL((advect_stream_a_kernel<T, 2>), ga, block, 0, st, u, bin, bout, fa, sb, dt, nwa);
+        hipLaunchKernelGGL((advect_stream_b_kernel<T, 2>), gb, block, 0, st, u, bin, bout, fa, sb, dt, nwb);
+    }
+    return hipGetLastError();
+}
+
 template <typename T> hipError_t prepare_soa(int nx, int ny, int nz)
 {
     const int n3[3] = {nx, ny, nz};
@@ -342,6 +417,8 @@ template <typename T> hipError_t launch_aos_soa(T *soa, T *aos, int ncomp, const
     template hipError_t launch_advect_soa<T>(const T *, const T *, T *, int, const SoaGrid &, const T *,           \
                                              hipStream_t);                                                          \
     template hipError_t prepare_soa<T>(int, int, int);                                                              \
+    template hipError_t launch_advect_stream<T>(const T *, const T *, T *, T *, int, const SoaGrid &, const T *,    \
+                                                hipStream_t);                                                       \
     template hipError_t launch_cfl_soa<T>(const T *, const T *, const SoaGrid &, int, unsigned long long *,        \
                                           hipStream_t);                                                             \
     template hipError_t launch_monitors_soa<T>(const T *, const T *, const SoaGrid &, double *, int, double *,     \

@@ -1,0 +1,204 @@
+// Column-streaming advectbyzx for the y and z frames of the SoA state.
+//
+// In those frames the pencils run along y or z and neighbouring pencils
+// along j are neighbouring x, contiguous in memory.  Here a wave holds 64
+// consecutive pencils, one per lane, and streams along the pencil row by
+// row: every load and store of a row is one coalesced 256-byte access (the
+// tile kernels read 16-byte rows there, one cache line per row, and were
+// bound by the L1 request rate).  tvdb (Heterogeneous_MHD/C/tvdb.c:10-91)
+// has no pencil-wide reduction, only a stencil of +-4 rows, so each lane
+// carries a window of 15 values and emits one result per row, four rows
+// behind the row it loads.
+//
+// The update is the reference's two halves (advectbyzx.c:14-23): pass A
+// advects b2 along i with the face velocity between pencils j-1 and j
+// (advectbyzxA1.c:34-56) and stores the new b2 and the EMF flux FA in a
+// temporary plane (the reference's `temp`); pass B advects b3 with the face
+// velocity between planes k-1 and k, takes the flux at the k+1 face from a
+// second stream, and updates b1 = (((b1 - FA_j) + FA_{j+1}) - FB_k) + FB_{k+1}
+// (A1.c:55, A2.c:23, B1.c:54, B2.c:23).  Every arithmetic expression is the
+// tile kernels' (mhd_core.h tvdb_pencil), evaluated in the same order.
+//
+// A pencil is cut into segments of S rows for parallelism; each segment
+// re-reads 4 rows on either side.
+#pragma once
+#include "mhd_soa.h"
+
+namespace mhd {
+
+template <typename T> __device__ __forceinline__ T face_vel(T ma, T mb, T ra, T rb) { return (ma + mb) / (ra + rb); }
+
+// One tvdb pencil streamed row by row.  Row t brings the face-velocity
+// operands (minus side a, plus side b: A1.c:34) and the field value at t;
+// the push returns bnew[t-4] and F[t-4] = flux[t-5], the shifted flux of the
+// CT update.  Valid from the ninth push on (the first eight fill the window).
+template <typename T> struct TvdbStream {
+    T vr2 = 0, vr1 = 0;                  // raw face velocity at t-2, t-1
+    T vg2 = 0;                           // smoothed velocity at t-2
+    T bw4 = 0, bw3 = 0, bw2 = 0, bw1 = 0;  // field at t-4 .. t-1
+    T bv2 = 0;                           // b*vg at t-2
+    T vh4 = 0, vh3 = 0;                  // vh at t-4, t-3
+    T f13 = 0;                           // half-step flux at t-3
+    T g5 = 0, g4 = 0, g3 = 0;            // vg*b1 at t-5 .. t-3
+    T fl5 = 0;                           // flux at t-5
+
+    __device__ __forceinline__ void push(T ma, T mb, T ra, T rb, T bt, T dt, T &bnew, T &F)
+    {
+        const T vr0 = face_vel(ma, mb, ra, rb);
+        // A1.c:41 vx = (vx[i-1] + vx[i+1] + 2.0*vx)/4.0 at i = t-1
+        const T vg1 = (T)(((double)(vr2 + vr0) + 2.0 * (double)vr1) / 4.0);
+        // tvdb.c:27 vh = 0.5*vg + 0.5*vg[i+1] at i = t-2; tvdb.c:33 b*vg at t-1
+        const T vh2 = (T)0.5 * vg2 + (T)0.5 * vg1;
+        const T bv1 = bw1 * vg1;
+        // tvdb.c:36-46 upwind half-step flux at t-2
+        const T f12 = vh2 > (T)0 ? bv2 : bv1;
+        // tvdb.c:51 b1 = b - (flux1 - flux1[i-1])*dt/2.0 at t-2
+        const T b12 = (T)((double)bw2 - (double)((f12 - f13) * dt) / 2.0);
+        const T g2 = vg2 * b12;
+        // tvdb.c:54-78 limited flux at i = t-4 (g[i-1..i+2] = g5, g4, g3, g2)
+        T w, wp, wm;
+        if (vh4 > (T)0) {
+            w = g4;
+            wp = (T)((double)(g3 - w) / 2.0);
+            wm = (T)((double)(w - g5) / 2.0);
+        } else {
+            w = g3;
+            wp = (T)((double)(w - g2) / 2.0);
+            wm = (T)((double)(g4 - w) / 2.0);
+        }
+        const T dw = vanleer<T>(wm, wp);
+        const T fl4 = (w + dw) * dt;
+        // tvdb.c:87 b = b - (flux - flux[i-1]) at t-4
+        bnew = bw4 - (fl4 - fl5);
+        F = fl5;
+        vr2 = vr1;
+        vr1 = vr0;
+        vg2 = vg1;
+        bw4 = bw3;
+        bw3 = bw2;
+        bw2 = bw1;
+        bw1 = bt;
+        bv2 = bv1;
+        vh4 = vh3;
+        vh3 = vh2;
+        f13 = f12;
+        g5 = g4;
+        g4 = g3;
+        g3 = g2;
+        fl5 = fl4;
+    }
+};
+
+// Geometry of a streamed sweep: frame 1 (i,j,k) = (y,x,z), frame 2 (z,x,y).
+struct StreamBox {
+    long N, si, sk;   // plane stride, row stride along i, stride along k
+    int ni, nj, nk;   // pencil length, pencils along j (= nx), along k
+    int ntj, nseg, S; // 64-pencil tiles along j, segments per pencil, rows per segment
+};
+
+// periodic row index; t lies in [-4, n + 4) and n may be as small as 2
+__device__ __forceinline__ int wrap_row(int t, int n)
+{
+    while (t < 0) t += n;
+    while (t >= n) t -= n;
+    return t;
+}
+
+// pass A: b2 along i with the j-face velocity; new b2 -> bout, FA -> fa
+template <typename T, int FR>
+__device__ __forceinline__ void advect_stream_a(const T *__restrict__ u, const T *__restrict__ bin,
+                                                T *__restrict__ bout, T *__restrict__ fa, const StreamBox &sb,
+                                                const T *__restrict__ dtp, int wave)
+{
+    const int jt = wave % sb.ntj, r = wave / sb.ntj, k = r % sb.nk, seg = r / sb.nk;
+    const int j = jt * 64 + (int)lane_id();
+    const bool valid = j < sb.nj;
+    const int jj = valid ? j : sb.nj - 1;
+    const int jm = jj == 0 ? sb.nj - 1 : jj - 1;
+    const long N = sb.N;
+    const T *rho = u, *mom = u + Comp<FR>::u[1] * N;
+    const T *b2 = bin + Comp<FR>::b[1] * N;
+    T *b2o = bout + Comp<FR>::b[1] * N;
+    const T dt = *dtp;
+    const int s0 = seg * sb.S, s1 = min(s0 + sb.S, sb.ni);
+    const long kb = (long)k * sb.sk;
+    TvdbStream<T> st;
+#pragma unroll 4
+    for (int t = s0 - 4; t < s1 + 4; t++) {
+        const long ro = (long)wrap_row(t, sb.ni) * sb.si + kb;
+        T bnew, F;
+        st.push(mom[ro + jm], mom[ro + jj], rho[ro + jm], rho[ro + jj], b2[ro + jj], dt, bnew, F);
+        if (t >= s0 + 4 && valid) {
+            const long oo = (long)(t - 4) * sb.si + kb + jj;
+            b2o[oo] = bnew;
+            fa[oo] = F;
+        }
+    }
+}
+
+// pass B: b3 along i with the k-face velocity; b1 from FA_j, FA_{j+1} (fa)
+// and FB_k, FB_{k+1}.  A lane holds the pencils of two neighbouring planes
+// k0 and k0+1 (one where k0 is the last plane of an odd count): three
+// streams at the faces k0, k0+1, k0+2 give both pencils' fluxes, the middle
+// one shared.
+template <typename T, int FR>
+__device__ __forceinline__ void advect_stream_b(const T *__restrict__ u, const T *__restrict__ bin,
+                                                T *__restrict__ bout, const T *__restrict__ fa, const StreamBox &sb,
+                                                const T *__restrict__ dtp, int wave)
+{
+    const int nk2 = (sb.nk + 1) / 2;
+    const int jt = wave % sb.ntj, r = wave / sb.ntj, kq = r % nk2, seg = r / nk2;
+    const int j = jt * 64 + (int)lane_id();
+    const bool valid = j < sb.nj;
+    const int jj = valid ? j : sb.nj - 1;
+    const int jp = jj + 1 == sb.nj ? 0 : jj + 1;
+    const long N = sb.N;
+    const T *rho = u, *mom = u + Comp<FR>::u[1] * N;
+    const T *b1 = bin + Comp<FR>::b[0] * N, *b3 = bin + Comp<FR>::b[2] * N;
+    T *b1o = bout + Comp<FR>::b[0] * N, *b3o = bout + Comp<FR>::b[2] * N;
+    const T dt = *dtp;
+    const int s0 = seg * sb.S, s1 = min(s0 + sb.S, sb.ni);
+    const int k0 = 2 * kq;
+    const bool two = k0 + 1 < sb.nk;  // uniform over the wave
+    const int km = k0 == 0 ? sb.nk - 1 : k0 - 1;
+    const int k1 = k0 + 1 == sb.nk ? 0 : k0 + 1;
+    const int k2 = k1 + 1 == sb.nk ? 0 : k1 + 1;
+    const long kmb = (long)km * sb.sk, k0b = (long)k0 * sb.sk, k1b = (long)k1 * sb.sk, k2b = (long)k2 * sb.sk;
+    TvdbStream<T> sa, sbk, sc;  // faces k0, k0+1, k0+2
+#pragma unroll 2
+    for (int t = s0 - 4; t < s1 + 4; t++) {
+        const long ri = (long)wrap_row(t, sb.ni) * sb.si + jj;
+        const T m0 = mom[ri + k0b], r0 = rho[ri + k0b], m1 = mom[ri + k1b], r1 = rho[ri + k1b];
+        T b3n0, F0, b3n1, F1;
+        sa.push(mom[ri + kmb], m0, rho[ri + kmb], r0, b3[ri + k0b], dt, b3n0, F0);
+        sbk.push(m0, m1, r0, r1, b3[ri + k1b], dt, b3n1, F1);
+        T F2 = 0;
+        if (two) {
+            T b3x;
+            sc.push(m1, mom[ri + k2b], r1, rho[ri + k2b], b3[ri + k2b], dt, b3x, F2);
+        }
+        if (t >= s0 + 4 && valid) {
+            const long row = (long)(t - 4) * sb.si;
+            const long o0 = row + k0b;
+            T v = b1[o0 + jj];
+            v = v - fa[o0 + jj];
+            v = v + fa[o0 + jp];
+            v = v - F0;
+            v = v + F1;
+            b1o[o0 + jj] = v;
+            b3o[o0 + jj] = b3n0;
+            if (two) {
+                const long o1 = row + k1b;
+                T w = b1[o1 + jj];
+                w = w - fa[o1 + jj];
+                w = w + fa[o1 + jp];
+                w = w - F1;
+                w = w + F2;
+                b1o[o1 + jj] = w;
+                b3o[o1 + jj] = b3n1;
+            }
+        }
+    }
+}
+
+}  // namespace mhd

@@ -134,7 +134,11 @@ def test_config1_alfven_32cube_200_steps(M, oracle):
 @pytest.mark.parametrize("shape,seed,amp,steps,tf", [((24, 16, 32), 501, 0.4, 5, math.inf),
                                                      ((16, 16, 16), 502, 0.55, 6, 3.0),
                                                      ((48, 96, 64), 503, 0.3, 2, math.inf),
-                                                     ((42, 22, 26), 504, 0.45, 3, math.inf)])
+                                                     ((42, 22, 26), 504, 0.45, 3, math.inf),
+                                                     # pencils of 2 and 3 cells; a partial 64-pencil tile
+                                                     ((70, 2, 3), 505, 0.4, 3, math.inf),
+                                                     # y/z pencils cut into segments, the last one short; an odd plane count
+                                                     ((64, 100, 45), 506, 0.35, 2, math.inf)])
 def test_ctx_steps_match_oracle(M, oracle, shape, seed, amp, steps, tf):
     nx, ny, nz = shape
     u0, b0 = random_state(nx, ny, nz, seed=seed, amp=amp)
