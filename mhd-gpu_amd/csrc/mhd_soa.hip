@@ -58,7 +58,9 @@ __global__ void __launch_bounds__(256) advect_stream_a_kernel(const T *__restric
                                                               T *__restrict__ bout, T *__restrict__ fa, StreamBox sb,
                                                               const T *__restrict__ dtp, int nwaves)
 {
-    const int wave = xcd_tile(blockIdx.x, gridDim.x) * 4 + (int)(threadIdx.x >> 6);
+    // the wave index is uniform: readfirstlane keeps every pencil index and
+    // row counter derived from it in scalar registers
+    const int wave = xcd_tile(blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (wave < nwaves) advect_stream_a<T, FR>(u, bin, bout, fa, sb, dtp, wave);
 }
 
@@ -67,7 +69,7 @@ __global__ void __launch_bounds__(256) advect_stream_b_kernel(const T *__restric
                                                               T *__restrict__ bout, const T *__restrict__ fa,
                                                               StreamBox sb, const T *__restrict__ dtp, int nwaves)
 {
-    const int wave = xcd_tile(blockIdx.x, gridDim.x) * 4 + (int)(threadIdx.x >> 6);
+    const int wave = xcd_tile(blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (wave < nwaves) advect_stream_b<T, FR>(u, bin, bout, fa, sb, dtp, wave);
 }
 

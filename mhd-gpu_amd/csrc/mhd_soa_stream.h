@@ -123,11 +123,21 @@ __device__ __forceinline__ void advect_stream_a(const T *__restrict__ u, const T
     const int s0 = seg * sb.S, s1 = min(s0 + sb.S, sb.ni);
     const long kb = (long)k * sb.sk;
     TvdbStream<T> st;
-#pragma unroll 4
+    // the operands of row t+1 are loaded while row t is computed
+    long ro = (long)wrap_row(s0 - 4, sb.ni) * sb.si + kb;
+    T ma = mom[ro + jm], mb = mom[ro + jj], ra = rho[ro + jm], rb = rho[ro + jj], bt = b2[ro + jj];
     for (int t = s0 - 4; t < s1 + 4; t++) {
-        const long ro = (long)wrap_row(t, sb.ni) * sb.si + kb;
+        const T cma = ma, cmb = mb, cra = ra, crb = rb, cbt = bt;
+        if (t + 1 < s1 + 4) {
+            ro = (long)wrap_row(t + 1, sb.ni) * sb.si + kb;
+            ma = mom[ro + jm];
+            mb = mom[ro + jj];
+            ra = rho[ro + jm];
+            rb = rho[ro + jj];
+            bt = b2[ro + jj];
+        }
         T bnew, F;
-        st.push(mom[ro + jm], mom[ro + jj], rho[ro + jm], rho[ro + jj], b2[ro + jj], dt, bnew, F);
+        st.push(cma, cmb, cra, crb, cbt, dt, bnew, F);
         if (t >= s0 + 4 && valid) {
             const long oo = (long)(t - 4) * sb.si + kb + jj;
             b2o[oo] = bnew;
@@ -165,17 +175,40 @@ __device__ __forceinline__ void advect_stream_b(const T *__restrict__ u, const T
     const int k2 = k1 + 1 == sb.nk ? 0 : k1 + 1;
     const long kmb = (long)km * sb.sk, k0b = (long)k0 * sb.sk, k1b = (long)k1 * sb.sk, k2b = (long)k2 * sb.sk;
     TvdbStream<T> sa, sbk, sc;  // faces k0, k0+1, k0+2
-#pragma unroll 2
-    for (int t = s0 - 4; t < s1 + 4; t++) {
+    // the operands of row t+1 are loaded while row t is computed
+    struct Row {
+        T mm, rm, m0, r0, m1, r1, m2, r2, b30, b31, b32;
+    };
+    auto load_row = [&](int t, Row &q) {
         const long ri = (long)wrap_row(t, sb.ni) * sb.si + jj;
-        const T m0 = mom[ri + k0b], r0 = rho[ri + k0b], m1 = mom[ri + k1b], r1 = rho[ri + k1b];
+        q.mm = mom[ri + kmb];
+        q.rm = rho[ri + kmb];
+        q.m0 = mom[ri + k0b];
+        q.r0 = rho[ri + k0b];
+        q.m1 = mom[ri + k1b];
+        q.r1 = rho[ri + k1b];
+        q.b30 = b3[ri + k0b];
+        q.b31 = b3[ri + k1b];
+        if (two) {
+            q.m2 = mom[ri + k2b];
+            q.r2 = rho[ri + k2b];
+            q.b32 = b3[ri + k2b];
+        } else {
+            q.m2 = q.r2 = q.b32 = (T)0;
+        }
+    };
+    Row nx;
+    load_row(s0 - 4, nx);
+    for (int t = s0 - 4; t < s1 + 4; t++) {
+        const Row q = nx;
+        if (t + 1 < s1 + 4) load_row(t + 1, nx);
         T b3n0, F0, b3n1, F1;
-        sa.push(mom[ri + kmb], m0, rho[ri + kmb], r0, b3[ri + k0b], dt, b3n0, F0);
-        sbk.push(m0, m1, r0, r1, b3[ri + k1b], dt, b3n1, F1);
+        sa.push(q.mm, q.m0, q.rm, q.r0, q.b30, dt, b3n0, F0);
+        sbk.push(q.m0, q.m1, q.r0, q.r1, q.b31, dt, b3n1, F1);
         T F2 = 0;
         if (two) {
             T b3x;
-            sc.push(m1, mom[ri + k2b], r1, rho[ri + k2b], b3[ri + k2b], dt, b3x, F2);
+            sc.push(q.m1, q.m2, q.r1, q.r2, q.b32, dt, b3x, F2);
         }
         if (t >= s0 + 4 && valid) {
             const long row = (long)(t - 4) * sb.si;
