@@ -120,8 +120,9 @@ int mhd_main(float *h_u, float *h_b, const int *nx, const int *ny, const int *nz
 typedef struct mhd_ctx mhd_ctx;
 
 /* precision: 4 (FP32, the reference's) or 8 (FP64).  avg_b as in mhd_cfl.
- * Allocates 44 B (FP32) or 88 B (FP64) per cell on the current device: the
- * state (32 B) plus the second field buffer of advectbyzx (12 B). */
+ * Allocates 48 B (FP32) or 96 B (FP64) per cell on the current device: the
+ * state (32 B), the second field buffer of advectbyzx (12 B) and the
+ * scratch plane of its streamed y/z passes (4 B). */
 int mhd_ctx_create(mhd_ctx **ctx, int nx, int ny, int nz, int precision, int avg_b);
 int mhd_ctx_destroy(mhd_ctx *ctx);
 /* host arrays in the ctx's precision (float* or double*) */

@@ -19,10 +19,14 @@ namespace mhd {
 
 struct BoxDesc {
     float *u, *b, *b2;   // SoA state and the second field buffer
+    float *fa;           // scratch plane of the streamed y/z advectbyzx
     SoaGrid g;
     SoaBox fr[3];        // the three sweep frames
     int K[3];            // cells per lane of each frame
     int ntj[3];          // tiles along j of each frame
+    StreamBox sb[3];     // streamed advectbyzx of frames 1 and 2
+    int nwa[3], nwb[3];  // its waves (pass A, pass B)
+    int streamed;        // the y/z advectbyzx of this box is streamed (schedule 3)
 };
 
 struct WorkItem {
@@ -41,6 +45,8 @@ struct BoxGroup {
     int cfl_off, cfl_n;
     int act_off, act_n;
     int lds_f[3], lds_a[3];  // dynamic LDS bytes of fluidx / advect per frame
+    int sa_off[3], sa_n[3], sb_off[3], sb_n[3];  // block items of the streamed y/z advectbyzx
+    int tile_adv_n[3];  // leading items of item_off[fr] whose boxes run the tile advectbyzx
     long cells;
 };
 
@@ -52,7 +58,7 @@ public:
     int init();  // allocations; MHD_OK or an error code
     hipStream_t stream() const { return st_; }
 
-    // slots: device state of one box (44 B per cell)
+    // slots: device state of one box (48 B per cell)
     int add_box(int nx, int ny, int nz, int *slot);
     int remove_box(int slot);
     int set_state(int slot, const float *h_u, const float *h_b);
@@ -87,7 +93,7 @@ public:
 
 private:
     struct Chunk {
-        float *base;  // 11 planes: u (5), b (3), b2 (3)
+        float *base;  // 12 planes: u (5), b (3), b2 (3), fa (1)
         long N;
     };
     int alloc_box(long N, float **base);
@@ -122,6 +128,8 @@ private:
     unsigned *d_ctr_ = nullptr;        // 12 sweeps x 8 queues
     int *d_part_ = nullptr;            // 3 frames x 9 partition bounds
     WorkItem *d_items_[3] = {nullptr, nullptr, nullptr};
+    WorkItem *d_sitems_[2] = {nullptr, nullptr};  // streamed advectbyzx blocks: pass A, pass B (both frames)
+    int cap_sitems_[2] = {0, 0};
     CflItem *d_cfl_ = nullptr;
     int nitems_[3] = {0, 0, 0}, ncfl_ = 0, cap_items_[3] = {0, 0, 0}, cap_cfl_ = 0;
     std::vector<BoxGroup> groups_;

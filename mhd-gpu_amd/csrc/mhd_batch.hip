@@ -7,6 +7,7 @@
 #include "../../include/mhd_api.h"
 #include "mhd_batch.h"
 #include "mhd_internal.h"
+#include "mhd_soa_stream.h"
 #include "mhd_soa_tiles.h"
 
 namespace mhd {
@@ -150,6 +151,34 @@ __global__ void __launch_bounds__(64 * kBW)
         advect_soa_tile<float, K, kBW, FR>(u, bcur, bnext, bd.fr[FR], dt + wi.box, wi.tile, bd.ntj[FR], smem);
     else
         fluidx_soa_tile<float, K, kBW, FR>(u, bcur, bd.fr[FR], dt + wi.box, wi.tile, bd.ntj[FR], smem);
+}
+
+// Streamed advectbyzx of the y/z frames (schedule 3): one block per item
+// (box, first wave), four waves; the field of the sweep is in b (flip 0) or b2.
+template <int FR>
+__global__ void __launch_bounds__(256) batch_stream_a_kernel(const BoxDesc *__restrict__ boxes,
+                                                             const WorkItem *__restrict__ items,
+                                                             const float *__restrict__ dt, int flip)
+{
+    const WorkItem wi = items[xcd_tile(blockIdx.x, gridDim.x)];
+    const BoxDesc &bd = boxes[wi.box];
+    const int wave = wi.tile + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (wave < bd.nwa[FR])
+        advect_stream_a<float, FR>(bd.u, flip ? bd.b2 : bd.b, flip ? bd.b : bd.b2, bd.fa, bd.sb[FR], dt + wi.box,
+                                   wave);
+}
+
+template <int FR>
+__global__ void __launch_bounds__(256) batch_stream_b_kernel(const BoxDesc *__restrict__ boxes,
+                                                             const WorkItem *__restrict__ items,
+                                                             const float *__restrict__ dt, int flip)
+{
+    const WorkItem wi = items[xcd_tile(blockIdx.x, gridDim.x)];
+    const BoxDesc &bd = boxes[wi.box];
+    const int wave = wi.tile + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (wave < bd.nwb[FR])
+        advect_stream_b<float, FR>(bd.u, flip ? bd.b2 : bd.b, flip ? bd.b : bd.b2, bd.fa, bd.sb[FR], dt + wi.box,
+                                   wave);
 }
 
 // CFL over the active boxes: one block per (box, unit range); per-box max
@@ -335,7 +364,7 @@ Engine::~Engine()
     for (float *x : xbuf_)
         if (x) (void)hipFree(x);
     void *ps[] = {d_boxes_, d_dt_, d_t_, d_tf_, d_dts_, d_t64_, d_cmax_, d_iter_, d_active_, d_ctr_, d_part_,
-                  d_items_[0], d_items_[1], d_items_[2], d_cfl_};
+                  d_items_[0], d_items_[1], d_items_[2], d_cfl_, d_sitems_[0], d_sitems_[1]};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (ev0_) (void)hipEventDestroy(ev0_);
@@ -421,7 +450,7 @@ long Engine::cells(int slot) const
     return (long)g.nx * g.ny * g.nz;
 }
 
-// Box memory: one allocation of 11 planes (u, b, b2).  Released boxes go to
+// Box memory: one allocation of 12 planes (u, b, b2, the EMF scratch fa).  Released boxes go to
 // a pool and are reused for boxes of N <= chunk <= 2N cells per plane, so a
 // serving loop turning requests over calls no hipFree (which waits for the
 // whole device).  On an allocation failure the pool is emptied and the
@@ -439,7 +468,7 @@ int Engine::alloc_box(long N, float **base)
             return MHD_OK;
         }
     }
-    const size_t bytes = 11 * (size_t)N * 4;
+    const size_t bytes = 12 * (size_t)N * 4;
     hipError_t e = hipMalloc((void **)base, bytes);
     if (e != hipSuccess) {
         std::vector<Chunk> drop;
@@ -477,11 +506,18 @@ int Engine::claim_slot(int nx, int ny, int nz, hipStream_t st, int *slot)
     d.u = base;
     d.b = base + 5 * N;
     d.b2 = base + 8 * N;
+    d.fa = base + 11 * N;
     for (int fr = 0; fr < 3; fr++) {
         d.fr[fr] = make_soa_box<float>(fr, d.g);
         d.K[fr] = pick_k(d.fr[fr].f.ni);
         d.ntj[fr] = (d.fr[fr].f.nj + kBW - 1) / kBW;
+        if (fr > 0) {
+            d.sb[fr] = make_stream_box(fr, d.g);
+            d.nwa[fr] = stream_waves_a(d.sb[fr]);
+            d.nwb[fr] = stream_waves_b(d.sb[fr]);
+        }
     }
+    d.streamed = stream_pays(d.sb[1]) && stream_pays(d.sb[2]);
     int s = -1;
     {
         std::lock_guard<std::mutex> lk(mu_);
@@ -739,6 +775,10 @@ int Engine::build()
             members[g].push_back(s);
             groups_[g].cells += cells(s);
         }
+        // within a group the boxes whose y/z advectbyzx runs as tiles come
+        // first, so that their advect items lead each frame's item range
+        for (auto &m : members)
+            std::stable_partition(m.begin(), m.end(), [&](int s) { return !boxes_[s].streamed; });
         std::vector<int> gi(groups_.size());
         for (size_t g = 0; g < gi.size(); g++) gi[g] = (int)g;
         std::stable_sort(gi.begin(), gi.end(), [&](int a, int b) { return groups_[a].cells > groups_[b].cells; });
@@ -778,6 +818,8 @@ int Engine::build()
                 items[fr].push_back({s, t});
                 cost.push_back((double)d.K[fr]);  // lane-slots per tile ~ K (W pencils x 64 lanes x K)
             }
+            if (gnext > 0 && !d.streamed)
+                groups_[gnext - 1].tile_adv_n[fr] = (int)items[fr].size() - groups_[gnext - 1].item_off[fr];
         }
         for (size_t g = 0; g < groups_.size(); g++)
             groups_[g].item_n[fr] =
@@ -804,6 +846,31 @@ int Engine::build()
         if (e == hipSuccess && n) e = hipMemcpyAsync(d_items_[fr], items[fr].data(), n * sizeof(WorkItem),
                                                       hipMemcpyHostToDevice, st_);
         if (e != hipSuccess) return hip_fail(e, "engine build items");
+    }
+    // streamed y/z advectbyzx (schedule 3): per group and frame, one item per
+    // block of four waves of each box, pass A and pass B in their own lists
+    std::vector<WorkItem> sitems[2];
+    for (size_t g = 0; g < groups_.size(); g++) {
+        BoxGroup &G = groups_[g];
+        for (int fr = 1; fr < 3; fr++) {
+            G.sa_off[fr] = (int)sitems[0].size();
+            G.sb_off[fr] = (int)sitems[1].size();
+            for (int oi = G.act_off; oi < G.act_off + G.act_n; oi++) {
+                const BoxDesc &d = boxes_[order[oi]];
+                if (!d.streamed) continue;
+                for (int w = 0; w < d.nwa[fr]; w += 4) sitems[0].push_back({order[oi], w});
+                for (int w = 0; w < d.nwb[fr]; w += 4) sitems[1].push_back({order[oi], w});
+            }
+            G.sa_n[fr] = (int)sitems[0].size() - G.sa_off[fr];
+            G.sb_n[fr] = (int)sitems[1].size() - G.sb_off[fr];
+        }
+    }
+    for (int p = 0; p < 2; p++) {
+        const int n = (int)sitems[p].size();
+        e = ensure_cap(&d_sitems_[p], &cap_sitems_[p], std::max(n, 1));
+        if (e == hipSuccess && n)
+            e = hipMemcpyAsync(d_sitems_[p], sitems[p].data(), n * sizeof(WorkItem), hipMemcpyHostToDevice, st_);
+        if (e != hipSuccess) return hip_fail(e, "engine build stream items");
     }
     // CFL items: ranges of 1024 units (4 per thread; a unit is a 16-byte
     // vector of a row, or a cell where rows are not vector-aligned)
@@ -871,8 +938,29 @@ int Engine::enqueue_step()
             for (int sw = 0; sw < 12 && e == hipSuccess; sw++) {
                 const int fr = kSweepFrameB[sw];
                 const bool adv = kSweepOpB[sw] == 1;
-                e = launch_group(fr, G.K[fr], adv, G.item_n[fr], adv ? G.lds_a[fr] : G.lds_f[fr], S, d_boxes_,
-                                 d_items_[fr] + G.item_off[fr], d_dt_, flip_before(sw));
+                const int flip = flip_before(sw);
+                if (adv && fr > 0) {
+                    // y/z frames: tiles for the small boxes, the streamed passes for the others
+                    if (G.tile_adv_n[fr] > 0)
+                        e = launch_group(fr, G.K[fr], true, G.tile_adv_n[fr], G.lds_a[fr], S, d_boxes_,
+                                         d_items_[fr] + G.item_off[fr], d_dt_, flip);
+                    if (e != hipSuccess || G.sa_n[fr] == 0) continue;
+                    if (fr == 1) {
+                        hipLaunchKernelGGL(batch_stream_a_kernel<1>, dim3(G.sa_n[1]), dim3(256), 0, S, d_boxes_,
+                                           d_sitems_[0] + G.sa_off[1], d_dt_, flip);
+                        hipLaunchKernelGGL(batch_stream_b_kernel<1>, dim3(G.sb_n[1]), dim3(256), 0, S, d_boxes_,
+                                           d_sitems_[1] + G.sb_off[1], d_dt_, flip);
+                    } else {
+                        hipLaunchKernelGGL(batch_stream_a_kernel<2>, dim3(G.sa_n[2]), dim3(256), 0, S, d_boxes_,
+                                           d_sitems_[0] + G.sa_off[2], d_dt_, flip);
+                        hipLaunchKernelGGL(batch_stream_b_kernel<2>, dim3(G.sb_n[2]), dim3(256), 0, S, d_boxes_,
+                                           d_sitems_[1] + G.sb_off[2], d_dt_, flip);
+                    }
+                    e = hipGetLastError();
+                } else {
+                    e = launch_group(fr, G.K[fr], adv, G.item_n[fr], adv ? G.lds_a[fr] : G.lds_f[fr], S, d_boxes_,
+                                     d_items_[fr] + G.item_off[fr], d_dt_, flip);
+                }
             }
         }
         for (int i = 0; i < ns && e == hipSuccess; i++) {

@@ -92,8 +92,9 @@ template <typename T> static hipError_t enqueue_sweep(mhd_ctx *c, int s)
     T *bnext = (T *)(field_buffer_before(s) ? c->b : c->b2);
     if (kSweepOp[s] == 0)
         return launch_fluidx_soa<T>((T *)c->u, bcur, fr, c->g, (const T *)c->dt, c->st);
-    // x frame: LDS tiles (pencils contiguous); y/z frames: column streaming
-    if (fr == 0) return launch_advect_soa<T>((const T *)c->u, bcur, bnext, fr, c->g, (const T *)c->dt, c->st);
+    // x frame, and y/z frames of small boxes: LDS tiles; else column streaming
+    if (fr == 0 || !stream_pays(make_stream_box(fr, c->g)))
+        return launch_advect_soa<T>((const T *)c->u, bcur, bnext, fr, c->g, (const T *)c->dt, c->st);
     return launch_advect_stream<T>((const T *)c->u, bcur, bnext, (T *)c->fa, fr, c->g, (const T *)c->dt, c->st);
 }
 

@@ -40,10 +40,11 @@ struct Request {
 
 size_t device_bytes(const Request &r)
 {
-    // state 32 B + second field buffer 12 B per cell (padded planes), plus the
-    // 20 B/cell transfer buffer held while the state is uploaded or read back
+    // state 32 B + second field buffer 12 B + EMF scratch 4 B per cell
+    // (padded planes), plus the 20 B/cell transfer buffer held while the
+    // state is uploaded or read back
     const size_t c = (size_t)r.nx * r.ny * r.nz;
-    return 44 * c + 20 * c + (1 << 20);
+    return 48 * c + 20 * c + (1 << 20);
 }
 
 }  // namespace

@@ -33,6 +33,23 @@ struct SoaBox {
 };
 template <typename T> SoaBox make_soa_box(int fr, const SoaGrid &g);
 
+// Geometry of a column-streamed advectbyzx (mhd_soa_stream.h): frame 1
+// (i,j,k) = (y,x,z), frame 2 (z,x,y)
+struct StreamBox {
+    long N, si, sk;   // plane stride, row stride along i, stride along k
+    int ni, nj, nk;   // pencil length, pencils along j (= nx), along k
+    int ntj, nseg, S; // 64-pencil tiles along j, segments per pencil, rows per segment
+};
+StreamBox make_stream_box(int fr, const SoaGrid &g);
+// waves of the two passes: A one pencil per lane, B two planes per lane
+inline int stream_waves_a(const StreamBox &sb) { return sb.ntj * sb.nk * sb.nseg; }
+inline int stream_waves_b(const StreamBox &sb) { return sb.ntj * ((sb.nk + 1) / 2) * sb.nseg; }
+// The streamed passes need many waves to hide their per-row latency: below
+// 2048 waves of pass A the LDS-tile kernel is faster (measured on MI355X:
+// 128^3, 1024 waves, 0.64 ms/step streamed against 0.52 with tiles; 192^3,
+// 3456 waves, 1.48 against 1.55).
+inline bool stream_pays(const StreamBox &sb) { return stream_waves_a(sb) >= 2048; }
+
 template <typename T>
 hipError_t launch_fluidx_soa(T *u, const T *b, int fr, const SoaGrid &g, const T *dt, hipStream_t st);
 // reads bin, writes bout (distinct buffers)

@@ -334,8 +334,7 @@ hipError_t launch_advect_stream(const T *u, const T *bin, T *bout, T *fa, int fr
 {
     if (fr != 1 && fr != 2) return hipErrorInvalidValue;
     const StreamBox sb = make_stream_box(fr, g);
-    const int nwa = sb.ntj * sb.nk * sb.nseg;              // pass A: one pencil per lane
-    const int nwb = sb.ntj * ((sb.nk + 1) / 2) * sb.nseg;  // pass B: two planes per lane
+    const int nwa = stream_waves_a(sb), nwb = stream_waves_b(sb);
     const dim3 ga((nwa + 3) / 4), gb((nwb + 3) / 4), block(256);
     if (fr == 1) {
         hipLaunchKernelGGL((advect_stream_a_kernel<T, 1>), ga, block, 0, st, u, bin, bout, fa, sb, dt, nwa);

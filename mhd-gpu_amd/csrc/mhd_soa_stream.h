@@ -89,13 +89,6 @@ template <typename T> struct TvdbStream {
     }
 };
 
-// Geometry of a streamed sweep: frame 1 (i,j,k) = (y,x,z), frame 2 (z,x,y).
-struct StreamBox {
-    long N, si, sk;   // plane stride, row stride along i, stride along k
-    int ni, nj, nk;   // pencil length, pencils along j (= nx), along k
-    int ntj, nseg, S; // 64-pencil tiles along j, segments per pencil, rows per segment
-};
-
 // periodic row index; t lies in [-4, n + 4) and n may be as small as 2
 __device__ __forceinline__ int wrap_row(int t, int n)
 {

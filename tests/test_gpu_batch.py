@@ -64,3 +64,22 @@ def test_batch_matches_oracle_and_autotune_restores(M, oracle):
         assert bitwise(bt.dts(i), do)
         assert same(ug, uo) and same(bg, bo)
     bt.close()
+
+
+def test_batch_streamed_and_tiled_boxes_equal_alone(M):
+    # 192^3 streams its y/z advectbyzx, 144^3 (same K = 3, one group) and
+    # 40x36x28 run it as tiles
+    dims = [(192, 192, 192), (144, 144, 144), (40, 36, 28)]
+    states = [random_state(*d, seed=950 + i, amp=0.3) for i, d in enumerate(dims)]
+    bt = M.Batch(dims)
+    bt.set_schedule(0, 3)
+    for i, (u, b) in enumerate(states):
+        bt.set_state(i, u, b)
+    bt.step(3)
+    for i, d in enumerate(dims):
+        print("box", i, d, flush=True)
+        ug, bg = bt.get_state(i)
+        ua, ba, da = alone(M, d, *states[i], 3)
+        assert bitwise(bt.dts(i), da)
+        assert bitwise(ug, ua) and bitwise(bg, ba)
+    bt.close()
