@@ -28,6 +28,8 @@ struct mhd_ctx {
     void *u, *b;               // state: 5 + 3 planes of ncell values (SoA, ctx precision)
     void *b2;                  // second field buffer: advectbyzx reads one, writes the other
     void *fa;                  // scratch plane: the EMF flux of the y/z advectbyzx (pass A -> pass B)
+    void *cfl_part;            // block maxima of the CFL speed fused into the step's last sweep
+    int nparts;
     unsigned long long *cmax;  // CFL max accumulator
     void *dt, *t;              // device scalars in ctx precision
     double *t64;
@@ -64,13 +66,25 @@ int soa_pad_y(int prec) { return env_pad("MHD_PAD_Y", 0, prec); }
 int soa_pad_z(int prec) { return env_pad("MHD_PAD_Z", 256 / prec, prec); }
 }  // namespace mhd
 
-template <typename T> static hipError_t enqueue_cfl_dt(mhd_ctx *c)
+// The CFL of the state a step starts from is the max of the block maxima
+// that the previous step's last sweep (fluidx x) wrote, and of the
+// accumulator, which a standalone CFL fills whenever the state was set from
+// outside (prime_cfl): the dt kernel takes both and clears the accumulator.
+template <typename T> static hipError_t enqueue_dt(mhd_ctx *c)
 {
-    hipError_t e = launch_cfl_soa<T>((const T *)c->u, (const T *)c->b, c->g, c->avg, c->cmax, c->st);
-    if (e == hipSuccess)
-        e = launch_dt<T>(c->cmax, (T *)c->dt, (T *)c->t, c->t64, (T)c->tf, (T *)c->dts, c->iter, c->cap, c->st);
+    return launch_dt_part<T>(c->cmax, (const T *)c->cfl_part, c->nparts, (T *)c->dt, (T *)c->t, c->t64, (T)c->tf,
+                             (T *)c->dts, c->iter, c->cap, c->st);
+}
+
+template <typename T> static hipError_t prime_cfl(mhd_ctx *c)
+{
+    hipError_t e = launch_fill_zero<T>((T *)c->cfl_part, c->nparts, c->st);
+    if (e == hipSuccess) e = launch_zero_u64(c->cmax, 1, c->st);
+    if (e == hipSuccess) e = launch_cfl_soa<T>((const T *)c->u, (const T *)c->b, c->g, c->avg, c->cmax, c->st);
     return e;
 }
+
+static hipError_t prime_cfl_any(mhd_ctx *c) { return c->prec == 4 ? prime_cfl<float>(c) : prime_cfl<double>(c); }
 
 // sweep s of the Strang sequence (HIP/cu/cuda_main.cu:99-121): frame and op
 static const int kSweepFrame[12] = {0, 0, 1, 1, 2, 2, 2, 2, 1, 1, 0, 0};
@@ -90,8 +104,13 @@ template <typename T> static hipError_t enqueue_sweep(mhd_ctx *c, int s)
     const int fr = kSweepFrame[s];
     T *bcur = (T *)(field_buffer_before(s) ? c->b2 : c->b);
     T *bnext = (T *)(field_buffer_before(s) ? c->b : c->b2);
-    if (kSweepOp[s] == 0)
+    if (kSweepOp[s] == 0) {
+        // the last sweep (fluidx, x frame) also forms the next step's CFL
+        if (s == 11)
+            return launch_fluidx_soa<T>((T *)c->u, bcur, fr, c->g, (const T *)c->dt, c->st, (T *)c->cfl_part,
+                                        c->avg ? 2 : 1);
         return launch_fluidx_soa<T>((T *)c->u, bcur, fr, c->g, (const T *)c->dt, c->st);
+    }
     // x frame, and y/z frames of small boxes: LDS tiles; else column streaming
     if (fr == 0 || !stream_pays(make_stream_box(fr, c->g)))
         return launch_advect_soa<T>((const T *)c->u, bcur, bnext, fr, c->g, (const T *)c->dt, c->st);
@@ -100,7 +119,7 @@ template <typename T> static hipError_t enqueue_sweep(mhd_ctx *c, int s)
 
 template <typename T> static hipError_t enqueue_step(mhd_ctx *c)
 {
-    hipError_t e = enqueue_cfl_dt<T>(c);
+    hipError_t e = enqueue_dt<T>(c);
     for (int s = 0; s < 12 && e == hipSuccess; s++) e = enqueue_sweep<T>(c, s);
     return e;
 }
@@ -172,10 +191,12 @@ int mhd_ctx_create(mhd_ctx **out, int nx, int ny, int nz, int precision, int avg
         return hip_fail(e, "hipStreamCreate");
     }
     const size_t NP = (size_t)c->g.N * P;
+    c->nparts = precision == 4 ? fluidx_tiles_x<float>(c->g) : fluidx_tiles_x<double>(c->g);
     struct A {
         void **p;
         size_t n;
     } al[] = {{&c->u, 5 * NP},   {&c->b, 3 * NP}, {&c->b2, 3 * NP}, {&c->fa, NP},
+              {&c->cfl_part, (size_t)c->nparts * P},
               {(void **)&c->cmax, 64},
               {&c->dt, 64},                 {&c->t, 64},               {(void **)&c->t64, 64},
               {&c->dts, (size_t)kDtCap * P}, {(void **)&c->iter, 64},  {(void **)&c->mon_part, 6 * 8 * 1024},
@@ -208,7 +229,7 @@ int mhd_ctx_destroy(mhd_ctx *c)
     if (!c) return MHD_OK;
     if (c->st) (void)hipStreamSynchronize(c->st);
     drop_graph(c);
-    void *ps[] = {c->u, c->b, c->b2, c->fa, c->cmax, c->dt, c->t, c->t64, c->dts, c->iter, c->mon_part, c->mon_out};
+    void *ps[] = {c->u, c->b, c->b2, c->fa, c->cfl_part, c->cmax, c->dt, c->t, c->t64, c->dts, c->iter, c->mon_part, c->mon_out};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -253,7 +274,11 @@ int mhd_ctx_set_state(mhd_ctx *c, const void *h_u, const void *h_b)
     if (!c || !h_u || !h_b) return fail(MHD_ERR_ARG, "mhd_ctx_set_state: null argument");
     hipError_t e = hipStreamSynchronize(c->st);
     if (e != hipSuccess) return hip_fail(e, "mhd_ctx_set_state");
-    return transfer_state(c, const_cast<void *>(h_u), const_cast<void *>(h_b), 1);
+    const int rc = transfer_state(c, const_cast<void *>(h_u), const_cast<void *>(h_b), 1);
+    if (rc) return rc;
+    e = prime_cfl_any(c);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    return e == hipSuccess ? MHD_OK : hip_fail(e, "mhd_ctx_set_state (CFL)");
 }
 
 int mhd_ctx_get_state(mhd_ctx *c, void *h_u, void *h_b)
@@ -275,6 +300,7 @@ int mhd_ctx_reset_clock(mhd_ctx *c, double tf)
     if (e == hipSuccess) e = hipMemsetAsync(c->dt, 0, 64, c->st);
     if (e == hipSuccess) e = hipMemsetAsync(c->t64, 0, 64, c->st);
     if (e == hipSuccess) e = hipMemsetAsync(c->iter, 0, 64, c->st);
+    if (e == hipSuccess) e = prime_cfl_any(c);
     if (e == hipSuccess) e = hipStreamSynchronize(c->st);
     return e == hipSuccess ? MHD_OK : hip_fail(e, "mhd_ctx_reset_clock");
 }
@@ -398,7 +424,7 @@ int mhd_ctx_profile_step(mhd_ctx *c, int reps, float *ms13)
     hipError_t e = hipSuccess;
     for (int r = 0; r < reps && e == hipSuccess; r++) {
         e = hipEventRecord(ev[0], c->st);
-        if (e == hipSuccess) e = c->prec == 4 ? enqueue_cfl_dt<float>(c) : enqueue_cfl_dt<double>(c);
+        if (e == hipSuccess) e = c->prec == 4 ? enqueue_dt<float>(c) : enqueue_dt<double>(c);
         if (e == hipSuccess) e = hipEventRecord(ev[1], c->st);
         for (int s = 0; s < 12 && e == hipSuccess; s++) {
             e = c->prec == 4 ? enqueue_sweep<float>(c, s) : enqueue_sweep<double>(c, s);

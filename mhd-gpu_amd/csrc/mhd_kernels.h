@@ -23,6 +23,11 @@ template <typename T>
 hipError_t launch_cfl(const T *u, const T *b, int nx, int ny, int nz, int avg, unsigned long long *cmax,
                       hipStream_t st);
 // dt, t, dt history from *cmax; clears *cmax
+// dt from block maxima of a fused CFL (part) and the accumulator; clears the accumulator
+template <typename T>
+hipError_t launch_dt_part(unsigned long long *cmax, const T *part, int np, T *dt, T *t, double *t64, T tf, T *dts,
+                          int *iter, int cap, hipStream_t st);
+template <typename T> hipError_t launch_fill_zero(T *p, int n, hipStream_t st);
 template <typename T>
 hipError_t launch_dt(unsigned long long *cmax, T *dt, T *t, double *t64, T tf, T *dts, int *iter, int cap,
                      hipStream_t st);

@@ -404,6 +404,49 @@ __global__ void dt_kernel(unsigned long long *cmax, T *dt, T *t, double *t64, T 
     *iter = it + 1;
 }
 
+// The same dt when the CFL max comes as block maxima (part[0..np)) of the
+// previous step's fused last sweep, beside the accumulator (which a
+// standalone CFL fills after the state was set from outside).  Deterministic:
+// max is order-independent.
+template <typename T>
+__global__ void __launch_bounds__(1024) dt_part_kernel(unsigned long long *cmax, const T *__restrict__ part, int np,
+                                                       T *dt, T *t, double *t64, T tf, T *dts, int *iter, int cap)
+{
+    // four independent loads in flight per thread
+    T c0 = (T)0, c1 = (T)0, c2 = (T)0, c3 = (T)0;
+    const int nt = blockDim.x;
+    int i = threadIdx.x;
+    for (; i + 3 * nt < np; i += 4 * nt) {
+        c0 = fmax(c0, part[i]);
+        c1 = fmax(c1, part[i + nt]);
+        c2 = fmax(c2, part[i + 2 * nt]);
+        c3 = fmax(c3, part[i + 3 * nt]);
+    }
+    for (; i < np; i += nt) c0 = fmax(c0, part[i]);
+    T c = wave_max(fmax(fmax(c0, c1), fmax(c2, c3)));
+    __shared__ T red[16];
+    if (lane_id() == 0) red[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int w = 1; w < nt / 64; w++) c = fmax(c, red[w]);
+    c = fmax(c, frombits<T>(*cmax));
+    *cmax = 0ull;
+    const T cfl = (T)(1.0 / (double)c);
+    T d = (T)(0.9 * (double)cfl);
+    if (d > (tf - *t) / (T)2) d = (tf - *t) / (T)2;
+    *t = *t + (T)2 * d;
+    *t64 = *t64 + 2.0 * (double)d;
+    *dt = d;
+    const int it = *iter;
+    if (it < cap) dts[it] = d;
+    *iter = it + 1;
+}
+
+template <typename T> __global__ void __launch_bounds__(256) fill_zero_kernel(T *p, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = (T)0;
+}
+
 // cfl value 1/c (calcfl.c:61) for the step-level entry point
 template <typename T> __global__ void cfl_value_kernel(unsigned long long *cmax, T *out)
 {
@@ -680,6 +723,21 @@ hipError_t launch_dt(unsigned long long *cmax, T *dt, T *t, double *t64, T tf, T
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_dt_part(unsigned long long *cmax, const T *part, int np, T *dt, T *t, double *t64, T tf, T *dts,
+                          int *iter, int cap, hipStream_t st)
+{
+    hipLaunchKernelGGL(dt_part_kernel<T>, dim3(1), dim3(1024), 0, st, cmax, part, np, dt, t, t64, tf, dts, iter, cap);
+    return hipGetLastError();
+}
+
+template <typename T> hipError_t launch_fill_zero(T *p, int n, hipStream_t st)
+{
+    const int nb = (n + 255) / 256;
+    hipLaunchKernelGGL(fill_zero_kernel<T>, dim3(nb < 1024 ? (nb > 0 ? nb : 1) : 1024), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
+
 template <typename T> hipError_t launch_cfl_value(unsigned long long *cmax, T *out, hipStream_t st)
 {
     hipLaunchKernelGGL(cfl_value_kernel<T>, dim3(1), dim3(1), 0, st, cmax, out);
@@ -738,6 +796,9 @@ hipError_t launch_monitors(const T *u, const T *b, int nx, int ny, int nz, doubl
     template hipError_t launch_advect<T>(const T *, const T *, T *, int, int, int, int, const T *, hipStream_t);              \
     template hipError_t launch_cfl<T>(const T *, const T *, int, int, int, int, unsigned long long *, hipStream_t); \
     template hipError_t launch_dt<T>(unsigned long long *, T *, T *, double *, T, T *, int *, int, hipStream_t);    \
+    template hipError_t launch_dt_part<T>(unsigned long long *, const T *, int, T *, T *, double *, T, T *, int *,   \
+                                          int, hipStream_t);                                                        \
+    template hipError_t launch_fill_zero<T>(T *, int, hipStream_t);                                                 \
     template hipError_t launch_cfl_value<T>(unsigned long long *, T *, hipStream_t);                               \
     template hipError_t launch_transpose<T>(T *, T *, const T *, const T *, int, int, int, int, hipStream_t);       \
     template hipError_t launch_monitors<T>(const T *, const T *, int, int, int, double *, int, double *, hipStream_t); \

@@ -50,8 +50,18 @@ inline int stream_waves_b(const StreamBox &sb) { return sb.ntj * ((sb.nk + 1) / 
 // 3456 waves, 1.48 against 1.55).
 inline bool stream_pays(const StreamBox &sb) { return stream_waves_a(sb) >= 2048; }
 
+// cflm (x frame only): 1 / 2 also writes the block maxima of the new state's
+// CFL speed (face / cell-averaged B) to cfl_part, one per tile (the x
+// frame's ntj * nk tiles)
 template <typename T>
-hipError_t launch_fluidx_soa(T *u, const T *b, int fr, const SoaGrid &g, const T *dt, hipStream_t st);
+hipError_t launch_fluidx_soa(T *u, const T *b, int fr, const SoaGrid &g, const T *dt, hipStream_t st,
+                             T *cfl_part = nullptr, int cflm = 0);
+// tiles of the x-frame fluidx (entries of its cfl_part)
+template <typename T> inline int fluidx_tiles_x(const SoaGrid &g)
+{
+    const int W = 16 / sizeof(T);
+    return (g.ny + W - 1) / W * g.nz;
+}
 // reads bin, writes bout (distinct buffers)
 template <typename T>
 hipError_t launch_advect_soa(const T *u, const T *bin, T *bout, int fr, const SoaGrid &g, const T *dt,

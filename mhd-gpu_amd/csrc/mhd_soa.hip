@@ -18,13 +18,14 @@
 
 namespace mhd {
 
-template <typename T, int K, int W, int FR, bool FASTDIV>
+template <typename T, int K, int W, int FR, bool FASTDIV, int CFLM = 0>
 __global__ void __launch_bounds__(64 * W)
-    fluidx_soa_kernel(T *__restrict__ u, const T *__restrict__ b, SoaBox bx, const T *__restrict__ dtp, int ntj)
+    fluidx_soa_kernel(T *__restrict__ u, const T *__restrict__ b, SoaBox bx, const T *__restrict__ dtp, int ntj,
+                      T *__restrict__ cfl_part)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    fluidx_soa_tile<T, K, W, FR, FASTDIV>(u, b, bx, dtp, xcd_tile(blockIdx.x, gridDim.x), ntj,
-                                          reinterpret_cast<T *>(smem_raw));
+    fluidx_soa_tile<T, K, W, FR, FASTDIV, CFLM>(u, b, bx, dtp, xcd_tile(blockIdx.x, gridDim.x), ntj,
+                                                reinterpret_cast<T *>(smem_raw), cfl_part);
 }
 
 // MHD_FAST_DIV=1 selects the fluidx build whose mhdflux divisions share
@@ -226,6 +227,13 @@ template <typename T, int K, int FR> static hipError_t soa_attr()
         a = hipFuncSetAttribute((const void *)fluidx_soa_kernel<T, K, W, FR, false>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         if (a != hipSuccess) return a;
+        if constexpr (FR == 0) {
+            for (const void *k : {(const void *)fluidx_soa_kernel<T, K, W, 0, false, 1>,
+                                  (const void *)fluidx_soa_kernel<T, K, W, 0, false, 2>}) {
+                a = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
+                if (a != hipSuccess) return a;
+            }
+        }
         return hipFuncSetAttribute((const void *)advect_soa_kernel<T, K, W, FR>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     }();
@@ -233,19 +241,33 @@ template <typename T, int K, int FR> static hipError_t soa_attr()
 }
 
 template <typename T, int K, int FR>
-static hipError_t fluidx_soa_k(T *u, const T *b, const SoaBox &bx, const T *dt, hipStream_t st)
+static hipError_t fluidx_soa_k(T *u, const T *b, const SoaBox &bx, const T *dt, hipStream_t st, T *cfl_part,
+                               int cflm)
 {
     constexpr int W = soa_w<T>();
     hipError_t e = soa_attr<T, K, FR>();
     if (e != hipSuccess) return e;
     const int ntj = (bx.f.nj + W - 1) / W;
     const size_t lds = sizeof(T) * fluidx_soa_words<W>(bx.f.ni);
+    const dim3 grid(ntj * bx.f.nk), block(64 * W);
+    if constexpr (FR == 0) {
+        if (cflm == 1) {
+            hipLaunchKernelGGL((fluidx_soa_kernel<T, K, W, 0, false, 1>), grid, block, lds, st, u, b, bx, dt, ntj,
+                               cfl_part);
+            return hipGetLastError();
+        }
+        if (cflm == 2) {
+            hipLaunchKernelGGL((fluidx_soa_kernel<T, K, W, 0, false, 2>), grid, block, lds, st, u, b, bx, dt, ntj,
+                               cfl_part);
+            return hipGetLastError();
+        }
+    }
     if (fast_div_enabled())
-        hipLaunchKernelGGL((fluidx_soa_kernel<T, K, W, FR, true>), dim3(ntj * bx.f.nk), dim3(64 * W), lds, st, u, b, bx,
-                           dt, ntj);
+        hipLaunchKernelGGL((fluidx_soa_kernel<T, K, W, FR, true>), grid, block, lds, st, u, b, bx, dt, ntj,
+                           (T *)nullptr);
     else
-        hipLaunchKernelGGL((fluidx_soa_kernel<T, K, W, FR, false>), dim3(ntj * bx.f.nk), dim3(64 * W), lds, st, u, b,
-                           bx, dt, ntj);
+        hipLaunchKernelGGL((fluidx_soa_kernel<T, K, W, FR, false>), grid, block, lds, st, u, b, bx, dt, ntj,
+                           (T *)nullptr);
     return hipGetLastError();
 }
 
@@ -263,13 +285,15 @@ static hipError_t advect_soa_k(const T *u, const T *bin, T *bout, const SoaBox &
 }
 
 template <typename T>
-hipError_t launch_fluidx_soa(T *u, const T *b, int fr, const SoaGrid &g, const T *dt, hipStream_t st)
+hipError_t launch_fluidx_soa(T *u, const T *b, int fr, const SoaGrid &g, const T *dt, hipStream_t st, T *cfl_part,
+                             int cflm)
 {
+    if (cflm && fr != 0) return hipErrorInvalidValue;
     const SoaBox bx = make_soa_box<T>(fr, g);
     const int K = pick_k(bx.f.ni);
-#define SF0(KK) fluidx_soa_k<T, KK, 0>(u, b, bx, dt, st)
-#define SF1(KK) fluidx_soa_k<T, KK, 1>(u, b, bx, dt, st)
-#define SF2(KK) fluidx_soa_k<T, KK, 2>(u, b, bx, dt, st)
+#define SF0(KK) fluidx_soa_k<T, KK, 0>(u, b, bx, dt, st, cfl_part, cflm)
+#define SF1(KK) fluidx_soa_k<T, KK, 1>(u, b, bx, dt, st, nullptr, 0)
+#define SF2(KK) fluidx_soa_k<T, KK, 2>(u, b, bx, dt, st, nullptr, 0)
     if (fr == 0) { SOA_KSWITCH(K, SF0) }
     if (fr == 1) { SOA_KSWITCH(K, SF1) }
     SOA_KSWITCH(K, SF2)
@@ -414,7 +438,8 @@ template <typename T> hipError_t launch_aos_soa(T *soa, T *aos, int ncomp, const
 
 #define SOA_INST(T)                                                                                                  \
     template SoaBox make_soa_box<T>(int, const SoaGrid &);                                                          \
-    template hipError_t launch_fluidx_soa<T>(T *, const T *, int, const SoaGrid &, const T *, hipStream_t);        \
+    template hipError_t launch_fluidx_soa<T>(T *, const T *, int, const SoaGrid &, const T *, hipStream_t, T *,   \
+                                             int);                                                                  \
     template hipError_t launch_advect_soa<T>(const T *, const T *, T *, int, const SoaGrid &, const T *,           \
                                              hipStream_t);                                                          \
     template hipError_t prepare_soa<T>(int, int, int);                                                              \

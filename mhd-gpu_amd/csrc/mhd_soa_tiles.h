@@ -121,6 +121,13 @@ __device__ __forceinline__ T cfl_range(const T *__restrict__ u, const T *__restr
     return c;
 }
 
+// per-wave maxima of the fused CFL (one block-wide array per kernel)
+template <typename T> __device__ __forceinline__ T *cfl_red_slot()
+{
+    __shared__ T red[16];
+    return red;
+}
+
 // ---------------------------------------------------------------------------
 // tile geometry
 // ---------------------------------------------------------------------------
@@ -156,10 +163,16 @@ template <int W> __device__ __forceinline__ TileCtx tile_ctx(const SoaBox &bx, i
 // ---------------------------------------------------------------------------
 template <int W> __host__ __device__ constexpr int fluidx_soa_words(int n) { return (9 * W + 1) * n; }
 
-template <typename T, int K, int W, int FR, bool FASTDIV = false>
+// CFLM (x frame only): 0 = plain sweep; 1 / 2 = also the CFL speed of the
+// new state (calcfl.c:45-56, face B / cell-averaged B of OpenMP/mhd.f90:
+// 112-120), the block's max written to cfl_part[tile] - the sweep is the
+// step's last, so that is the next step's CFL
+template <typename T, int K, int W, int FR, bool FASTDIV = false, int CFLM = 0>
 __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__restrict__ b, const SoaBox &bx,
-                                                const T *__restrict__ dtp, int tile, int ntj, T *smem)
+                                                const T *__restrict__ dtp, int tile, int ntj, T *smem,
+                                                T *__restrict__ cfl_part = nullptr)
 {
+    static_assert(CFLM == 0 || FR == 0, "the CFL is fused into the x sweep only");
     constexpr int VE = Vec16<T>::n;
     static_assert(W == VE, "tile width is one 16-byte vector");
     const Frame &f = bx.f;
@@ -328,6 +341,45 @@ __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__re
         }
     }
     __syncthreads();
+    if constexpr (CFLM != 0) {
+        // CFL speed of the new state: after the barrier (a scheduling
+        // boundary, so the sweep's registers are free) from the new u in LDS;
+        // shadow lanes repeat lane l % L and leave the max unchanged
+        T c = (T)0;
+        const int w = threadIdx.x >> 6;
+        if (w < ns) {
+            const int lc = lane_id() % L;
+#pragma unroll
+            for (int q = 0; q < K; q++) {
+                const int i = lc * K + q;
+                const T *ui = su + w * 5 * n + i;
+                const T b1 = sb[(w * 3 + 0) * n + i], b2 = sb[(w * 3 + 1) * n + i], b3 = sb[(w * 3 + 2) * n + i];
+                T B;
+                if constexpr (CFLM == 2) {
+                    const int ip = i + 1 == n ? 0 : i + 1;
+                    const T b1p = sb[(w * 3 + 0) * n + ip];
+                    const T b2p = w + 1 < ns ? sb[((w + 1) * 3 + 1) * n + i] : sb2n[i];
+                    const T b3p = sbk[w * n + i];
+                    const T bxa = (T)((double)(b1 + b1p) / 2.0);
+                    const T bya = (T)((double)(b2 + b2p) / 2.0);
+                    const T bza = (T)((double)(b3 + b3p) / 2.0);
+                    B = bxa * bxa + bya * bya + bza * bza;
+                } else {
+                    B = b1 * b1 + b2 * b2 + b3 * b3;
+                }
+                const T sp = cfl_speed<T>(ui[0], ui[n], ui[2 * n], ui[3 * n], ui[4 * n], B);
+                c = c > sp ? c : sp;  // max_2num (calcfl.c:55)
+            }
+        }
+        c = wave_max(c);
+        if (lane_id() == 0) cfl_red_slot<T>()[w] = c;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            T m = cfl_red_slot<T>()[0];
+            for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = fmax(m, cfl_red_slot<T>()[i]);
+            cfl_part[tile] = m;
+        }
+    }
 
     // store the new u of the tile's pencils
     if (FR == 0 && t.vec) {
