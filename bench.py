@@ -308,7 +308,9 @@ def main():
     # per-kernel times with HIP events on the context's stream
     prof = sim.profile_step(reps=5)
     sweep_ms = prof[1:]
-    fl = [sweep_ms[i] for i, s in enumerate(SWEEPS) if s.startswith("fluidx")]
+    # the last sweep (fluidx x) also forms the next step's CFL: the plain
+    # fluidx average is over the other five
+    fl = [sweep_ms[i] for i, s in enumerate(SWEEPS) if s.startswith("fluidx") and i != 11]
     ad = [sweep_ms[i] for i, s in enumerate(SWEEPS) if s.startswith("advect")]
     fl_ms = float(np.mean(fl))
     ad_ms = float(np.mean(ad))
@@ -342,7 +344,8 @@ def main():
         "step_roofline": {"bytes_per_zone_update": STEP_BYTES_PER_CELL,
                           "achieved_GBs": zus * STEP_BYTES_PER_CELL / 1e9,
                           "frac": zus * STEP_BYTES_PER_CELL / 1e9 / HBM_PEAK_GBS},
-        "kernel_ms": {"cfl_dt": float(prof[0]), "fluidx_avg": fl_ms, "advect_avg": ad_ms,
+        "kernel_ms": {"dt": float(prof[0]), "fluidx_avg": fl_ms, "fluidx_x_with_cfl": float(sweep_ms[11]),
+                      "advect_avg": ad_ms,
                       "sweeps": {f"{i:02d}_{s}": float(v) for i, (s, v) in enumerate(zip(SWEEPS, sweep_ms))}},
         "monitors_fp64": {"mass": m[0], "energy": m[1], "max_divB": m[5], "t": t, "t64": t64, "finite": finite},
     }

@@ -151,9 +151,10 @@ int mhd_ctx_device_state(mhd_ctx *ctx, void **d_u, void **d_b);
  * apart (sy >= nx, sz >= sy*ny: padded rows/planes avoid power-of-two strides) */
 int mhd_ctx_layout(mhd_ctx *ctx, long *sy, long *sz, long *plane);
 /* time per kernel of one step, for the roofline: runs `reps` steps with
- * events around every launch; ms_out[0..12] = cfl+dt, then the 12 sweeps in
+ * events around every launch; ms_out[0..12] = dt, then the 12 sweeps in
  * order (fluidx x, advect x, fluidx y, advect y, fluidx z, advect z,
- * advect z, fluidx z, advect y, fluidx y, advect x, fluidx x), averaged */
+ * advect z, fluidx z, advect y, fluidx y, advect x, fluidx x), averaged.
+ * The CFL reduction of the next step is fused into the last sweep. */
 int mhd_ctx_profile_step(mhd_ctx *ctx, int reps, float *ms_out13);
 
 /* ---- batched form: many independent boxes of uneven sizes ----------------
