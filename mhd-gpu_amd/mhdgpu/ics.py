@@ -8,7 +8,8 @@ b: 3 per cell on the left faces, x fastest), periodic, gamma = 5/3.
   b_y = -(dv_y(i) + dv_y(i-1))/2 on the faces (OpenMP/mhd.f90:168-184).
 * ``fixture_alfven`` - the reference's committed fixture content
   (data/source_init_alfvenlinear_*cube.dat, OpenMP/mhd.f90:174-184, with its
-  literal 3.14159 for pi), in plain x-fastest order.
+  literal 3.14159 for pi), in plain x-fastest order; ``write_fixture``
+  writes it in the fixture file format.
 * ``random_phase``   - the Alfven background plus a Gaussian random-phase
   velocity field, P(k) ~ k^-2 for 1 <= |k| <= 8, rms |dv| = 0.1 (Mach 0.1 for
   c_s = 1), and dB = curl A on the faces from a random vector potential of the
@@ -79,6 +80,14 @@ def fixture_alfven(n):
     rec[..., 5] = 1
     rec[..., 6] = b2[None, None, :]
     return rec.ravel()
+
+
+def write_fixture(path, n):
+    """Write the reference's fixture format for an n^3 box (Fortran stream of
+    8 float32 per cell, x fastest: OpenMP_offloadMIC/work.f90:127-132) with
+    the fixture content of fixture_alfven; sizes the reference does not ship
+    (64^3, 128^3) come out the same way as the 16^3 and 32^3 files."""
+    fixture_alfven(n).astype("<f4").tofile(path)
 
 
 def _kgrid(n):
