@@ -26,7 +26,7 @@ struct BoxDesc {
     int ntj[3];          // tiles along j of each frame
     StreamBox sb[3];     // streamed advectbyzx of frames 1 and 2
     int nwa[3], nwb[3];  // its waves (pass A, pass B)
-    int streamed;        // the y/z advectbyzx of this box is streamed (schedule 3)
+    int streamed;        // the streamed y/z advectbyzx pays for this box alone
 };
 
 struct WorkItem {
@@ -133,6 +133,7 @@ private:
     CflItem *d_cfl_ = nullptr;
     int nitems_[3] = {0, 0, 0}, ncfl_ = 0, cap_items_[3] = {0, 0, 0}, cap_cfl_ = 0;
     std::vector<BoxGroup> groups_;
+    std::vector<char> stream_now_;  // per slot: the y/z advectbyzx streams in the current build
     static constexpr int kAux = 4;  // side streams of the grouped schedule
     hipStream_t aux_[kAux] = {};
     hipEvent_t evfork_ = nullptr, evjoin_[kAux] = {};

@@ -775,10 +775,22 @@ int Engine::build()
             members[g].push_back(s);
             groups_[g].cells += cells(s);
         }
-        // within a group the boxes whose y/z advectbyzx runs as tiles come
-        // first, so that their advect items lead each frame's item range
-        for (auto &m : members)
-            std::stable_partition(m.begin(), m.end(), [&](int s) { return !boxes_[s].streamed; });
+        // The streamed y/z advectbyzx needs many waves in flight: a box streams
+        // if it has 2048 waves of its own or if its group has them together
+        // (the group's boxes run in one launch).  Within a group the boxes that
+        // run tiles come first, so that their advect items lead each frame's
+        // item range.
+        stream_now_.assign(max_boxes_, 0);
+        for (auto &m : members) {
+            long w1 = 0, w2 = 0;
+            for (int s : m) {
+                w1 += boxes_[s].nwa[1];
+                w2 += boxes_[s].nwa[2];
+            }
+            const bool together = w1 >= 2048 && w2 >= 2048;
+            for (int s : m) stream_now_[s] = together || boxes_[s].streamed;
+            std::stable_partition(m.begin(), m.end(), [&](int s) { return !stream_now_[s]; });
+        }
         std::vector<int> gi(groups_.size());
         for (size_t g = 0; g < gi.size(); g++) gi[g] = (int)g;
         std::stable_sort(gi.begin(), gi.end(), [&](int a, int b) { return groups_[a].cells > groups_[b].cells; });
@@ -818,7 +830,7 @@ int Engine::build()
                 items[fr].push_back({s, t});
                 cost.push_back((double)d.K[fr]);  // lane-slots per tile ~ K (W pencils x 64 lanes x K)
             }
-            if (gnext > 0 && !d.streamed)
+            if (gnext > 0 && !stream_now_[s])
                 groups_[gnext - 1].tile_adv_n[fr] = (int)items[fr].size() - groups_[gnext - 1].item_off[fr];
         }
         for (size_t g = 0; g < groups_.size(); g++)
@@ -857,7 +869,7 @@ int Engine::build()
             G.sb_off[fr] = (int)sitems[1].size();
             for (int oi = G.act_off; oi < G.act_off + G.act_n; oi++) {
                 const BoxDesc &d = boxes_[order[oi]];
-                if (!d.streamed) continue;
+                if (!stream_now_[order[oi]]) continue;
                 for (int w = 0; w < d.nwa[fr]; w += 4) sitems[0].push_back({order[oi], w});
                 for (int w = 0; w < d.nwb[fr]; w += 4) sitems[1].push_back({order[oi], w});
             }

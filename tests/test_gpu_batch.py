@@ -67,9 +67,11 @@ def test_batch_matches_oracle_and_autotune_restores(M, oracle):
 
 
 def test_batch_streamed_and_tiled_boxes_equal_alone(M):
-    # 192^3 streams its y/z advectbyzx, 144^3 (same K = 3, one group) and
-    # 40x36x28 run it as tiles
-    dims = [(192, 192, 192), (144, 144, 144), (40, 36, 28)]
+    # the y/z advectbyzx streams for a box with 2048 waves of its own (192^3)
+    # and for the boxes of a group that has them together: 144^3 beside it
+    # (K = 3), sixteen 64^3 boxes and 40x36x28 (K = 1); 96^3, alone in its
+    # group (K = 2), runs tiles.  Alone, every box but 192^3 runs tiles.
+    dims = [(192, 192, 192), (144, 144, 144), (96, 96, 96)] + [(64, 64, 64)] * 16 + [(40, 36, 28)]
     states = [random_state(*d, seed=950 + i, amp=0.3) for i, d in enumerate(dims)]
     bt = M.Batch(dims)
     bt.set_schedule(0, 3)
