@@ -156,6 +156,31 @@ def test_ctx_steps_match_oracle(M, oracle, shape, seed, amp, steps, tf):
     sim.close()
 
 
+@pytest.mark.parametrize("avg", [False, True])
+def test_set_state_between_steps_recomputes_the_cfl(M, avg):
+    # the CFL of a step comes from the previous step's last sweep; a state set
+    # in between must not inherit it: the run continues exactly as a fresh
+    # context started from that state
+    shape = (24, 20, 16)
+    ua, ba = random_state(*shape, seed=701, amp=0.5)
+    ub, bb = random_state(*shape, seed=702, amp=0.05)  # slower state: a larger dt
+    sim = M.Simulation(*shape, avg_b=avg)
+    sim.set_state(ua, ba)
+    sim.step(3)
+    sim.set_state(ub, bb)  # no reset_clock: set_state alone must recompute the CFL
+    sim.step(2)
+    u1, b1 = sim.get_state()
+    d1 = sim.dts()[3:]
+    sim.close()
+    fresh = M.Simulation(*shape, avg_b=avg)
+    fresh.set_state(ub, bb)
+    fresh.step(2)
+    u2, b2 = fresh.get_state()
+    d2 = fresh.dts()
+    fresh.close()
+    assert bitwise(d1, d2) and bitwise(u1, u2) and bitwise(b1, b2)
+
+
 def test_ctx_fp64_matches_fp64_oracle(M, oracle):
     n = 12
     u0, b0 = random_state(n, n, n, seed=601, amp=0.3)
