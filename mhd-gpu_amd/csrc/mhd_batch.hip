@@ -147,10 +147,17 @@ __global__ void __launch_bounds__(64 * kBW)
     float *const u = bd.u;
     const float *bcur = flip ? bd.b2 : bd.b;
     float *bnext = flip ? bd.b : bd.b2;
-    if constexpr (ADV)
+    if constexpr (ADV && FR == 0) {
+        // x frame: direct loads, LDS for the fluxes only
+        if (K % 4 == 0 && bd.fr[0].vec_ok)
+            advect_x_tile<float, K, kBW, K % 4 == 0>(u, bcur, bnext, bd.fr[0], dt + wi.box, wi.tile, bd.ntj[0], smem);
+        else
+            advect_x_tile<float, K, kBW, false>(u, bcur, bnext, bd.fr[0], dt + wi.box, wi.tile, bd.ntj[0], smem);
+    } else if constexpr (ADV) {
         advect_soa_tile<float, K, kBW, FR>(u, bcur, bnext, bd.fr[FR], dt + wi.box, wi.tile, bd.ntj[FR], smem);
-    else
+    } else {
         fluidx_soa_tile<float, K, kBW, FR>(u, bcur, bd.fr[FR], dt + wi.box, wi.tile, bd.ntj[FR], smem);
+    }
 }
 
 // Streamed advectbyzx of the y/z frames (schedule 3): one block per item
@@ -819,12 +826,14 @@ int Engine::build()
             const int nt = d.ntj[fr] * d.fr[fr].f.nk;
             kmax_ = std::max(kmax_, d.K[fr]);
             const int lf = (int)(4 * (9 * kBW + 1) * d.fr[fr].f.ni), la = (int)(4 * (7 * kBW + 2) * d.fr[fr].f.ni);
+            // schedule 3 runs the x-frame advectbyzx with direct loads (smaller LDS)
+            const int lag = fr == 0 ? (int)(4 * advect_x_words<kBW>(d.fr[fr].f.ni)) : la;
             lds_fluidx_ = std::max(lds_fluidx_, lf);
             lds_advect_ = std::max(lds_advect_, la);
             if (gnext > 0) {
                 BoxGroup &G = groups_[gnext - 1];
                 G.lds_f[fr] = std::max(G.lds_f[fr], lf);
-                G.lds_a[fr] = std::max(G.lds_a[fr], la);
+                G.lds_a[fr] = std::max(G.lds_a[fr], lag);
             }
             for (int t = 0; t < nt; t++) {
                 items[fr].push_back({s, t});

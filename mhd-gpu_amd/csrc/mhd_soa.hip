@@ -52,6 +52,16 @@ __global__ void __launch_bounds__(64 * W)
                                  reinterpret_cast<T *>(smem_raw));
 }
 
+template <typename T, int K, int W, bool VEC>
+__global__ void __launch_bounds__(64 * W)
+    advect_x_kernel(const T *__restrict__ u, const T *__restrict__ bin, T *__restrict__ bout, SoaBox bx,
+                    const T *__restrict__ dtp, int ntj)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    advect_x_tile<T, K, W, VEC>(u, bin, bout, bx, dtp, xcd_tile(blockIdx.x, gridDim.x), ntj,
+                                reinterpret_cast<T *>(smem_raw));
+}
+
 // column-streaming advectbyzx of the y and z frames (mhd_soa_stream.h):
 // four independent waves per block, blocks dealt XCD-contiguously
 template <typename T, int FR>
@@ -275,6 +285,19 @@ template <typename T, int K, int FR>
 static hipError_t advect_soa_k(const T *u, const T *bin, T *bout, const SoaBox &bx, const T *dt, hipStream_t st)
 {
     constexpr int W = soa_w<T>();
+    if constexpr (FR == 0) {
+        // x frame: direct loads, LDS only for the fluxes
+        const int ntj = (bx.f.nj + W - 1) / W;
+        const size_t lds = sizeof(T) * advect_x_words<W>(bx.f.ni);
+        constexpr bool kvec = K % (16 / sizeof(T)) == 0;
+        if (kvec && bx.vec_ok)
+            hipLaunchKernelGGL((advect_x_kernel<T, K, W, kvec>), dim3(ntj * bx.f.nk), dim3(64 * W), lds, st, u, bin,
+                               bout, bx, dt, ntj);
+        else
+            hipLaunchKernelGGL((advect_x_kernel<T, K, W, false>), dim3(ntj * bx.f.nk), dim3(64 * W), lds, st, u, bin,
+                               bout, bx, dt, ntj);
+        return hipGetLastError();
+    }
     hipError_t e = soa_attr<T, K, FR>();
     if (e != hipSuccess) return e;
     const int ntj = (bx.f.nj + W - 1) / W;

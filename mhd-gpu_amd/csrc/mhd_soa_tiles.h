@@ -676,4 +676,162 @@ __device__ __forceinline__ void advect_soa_tile(const T *__restrict__ u, const T
     }
 }
 
+// ---------------------------------------------------------------------------
+// advectbyzx in the x frame with direct loads: the pencils are contiguous,
+// so each wave reads the rows of its tvdb items straight from global memory
+// (lane l holds cells l*K .. l*K+K-1, coalesced) and only the EMF fluxes go
+// through LDS, for the b1 update that combines the items of four waves.
+// Same items, arithmetic and update order as advect_soa_tile.
+// LDS: X[3W+1][n] (the shifted fluxes of the items)
+// ---------------------------------------------------------------------------
+template <int W> __host__ __device__ constexpr int advect_x_words(int n) { return (3 * W + 1) * n; }
+
+template <typename T, int K>
+__device__ __forceinline__ void load_cells(const T *__restrict__ p, T (&o)[K])
+{
+    constexpr int VE = Vec16<T>::n;
+    if constexpr (K % VE == 0) {
+#pragma unroll
+        for (int v = 0; v < K / VE; v++) {
+            const V16<T> x = *reinterpret_cast<const V16<T> *>(p + v * VE);
+#pragma unroll
+            for (int e = 0; e < VE; e++) o[v * VE + e] = vget<T>(x, e);
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < K; q++) o[q] = p[q];
+    }
+}
+
+template <typename T, int K>
+__device__ __forceinline__ void store_cells(T *__restrict__ p, const T (&o)[K])
+{
+    constexpr int VE = Vec16<T>::n;
+    if constexpr (K % VE == 0 && sizeof(T) == 4) {
+#pragma unroll
+        for (int v = 0; v < K / VE; v++)
+            *reinterpret_cast<float4 *>(p + v * VE) = make_float4(o[v * VE], o[v * VE + 1], o[v * VE + 2], o[v * VE + 3]);
+    } else if constexpr (K % VE == 0) {
+#pragma unroll
+        for (int v = 0; v < K / VE; v++)
+            *reinterpret_cast<double2 *>(p + v * VE) = make_double2(o[v * VE], o[v * VE + 1]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < K; q++) p[q] = o[q];
+    }
+}
+
+// vec: 16-byte loads apply (rows and planes aligned, K a multiple of the
+// vector width); otherwise the cells are read one by one
+template <typename T, int K, int W, bool VEC>
+__device__ __forceinline__ void advect_x_tile(const T *__restrict__ u, const T *__restrict__ bin,
+                                              T *__restrict__ bout, const SoaBox &bx, const T *__restrict__ dtp,
+                                              int tile, int ntj, T *X)
+{
+    constexpr int FR = 0;
+    const Frame &f = bx.f;
+    const TileCtx t = tile_ctx<W>(bx, tile, ntj);
+    const int n = t.n, ns = t.ns, L = n / K;
+    const long N = bx.N;
+    const int jn = wrapn(t.j0 + ns, f.nj), jm = wrapn(t.j0 - 1 + f.nj, f.nj);
+    const T *rho = u, *mom = u + Comp<FR>::u[1] * N;
+    const T *b1i = bin, *b2i = bin + N, *b3i = bin + 2 * N;
+    T *b1o = bout, *b2o = bout + N, *b3o = bout + 2 * N;
+    const int nrow = 3 * ns + 1;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), nw = blockDim.x >> 6;
+    const int l = lane_id(), lc = l % L;
+    const int src_next = lc + 1 == L ? 0 : lc + 1, src_prev = lc == 0 ? L - 1 : lc - 1;
+    const long cell0 = (long)lc * K;
+    const T dt = *dtp;
+    for (int it = w; it < nrow; it += nw) {
+        // the item's two pencils (minus side first) and its field row
+        long pa, pb, pf;
+        const T *fld;
+        if (it <= ns) {  // FA: b2 of pencil j0+it with the face velocity from j-1, j
+            pa = (long)(it == 0 ? jm : t.j0 + it - 1) * f.sj + t.kb;
+            pb = (long)(it == ns ? jn : t.j0 + it) * f.sj + t.kb;
+            pf = pb;
+            fld = b2i;
+        } else if (it <= 2 * ns) {  // FB at face k: b3 of pencil j0+s
+            const long js = (long)(t.j0 + it - ns - 1) * f.sj;
+            pa = js + t.kmb;
+            pb = js + t.kb;
+            pf = pb;
+            fld = b3i;
+        } else {  // FB at face k+1: b3 of pencil j0+s at k+1 (flux only)
+            const long js = (long)(t.j0 + it - 2 * ns - 1) * f.sj;
+            pa = js + t.kb;
+            pb = js + t.kpb;
+            pf = pb;
+            fld = b3i;
+        }
+        T ma[K], mb[K], ra[K], rb[K], bi[K];
+        if constexpr (VEC) {
+            load_cells<T, K>(mom + pa + cell0, ma);
+            load_cells<T, K>(mom + pb + cell0, mb);
+            load_cells<T, K>(rho + pa + cell0, ra);
+            load_cells<T, K>(rho + pb + cell0, rb);
+            load_cells<T, K>(fld + pf + cell0, bi);
+        } else {
+#pragma unroll
+            for (int q = 0; q < K; q++) {
+                ma[q] = mom[pa + cell0 + q];
+                mb[q] = mom[pb + cell0 + q];
+                ra[q] = rho[pa + cell0 + q];
+                rb[q] = rho[pb + cell0 + q];
+                bi[q] = fld[pf + cell0 + q];
+            }
+        }
+        T vx[K], vxm[K], vxn[K], vs[K], bnew[K], F[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) vx[q] = face_v(ma[q], mb[q], ra[q], rb[q]);
+        shift_prev<T, K>(vx, vxm, src_prev);
+        shift_next<T, K>(vx, vxn, src_next);
+#pragma unroll
+        for (int q = 0; q < K; q++) vs[q] = (T)(((double)(vxm[q] + vxn[q]) + 2.0 * (double)vx[q]) / 4.0);
+        tvdb_pencil<T, K>(bi, vs, dt, bnew, F, src_next, src_prev);
+        if (l < L) {
+#pragma unroll
+            for (int q = 0; q < K; q++) X[it * n + cell0 + q] = F[q];
+            // new b2 (FA) / b3 (FB at k) of the tile's own pencils
+            if (it < ns) {
+                if constexpr (VEC) store_cells<T, K>(b2o + pf + cell0, bnew);
+                else
+#pragma unroll
+                    for (int q = 0; q < K; q++) b2o[pf + cell0 + q] = bnew[q];
+            } else if (it > ns && it <= 2 * ns) {
+                if constexpr (VEC) store_cells<T, K>(b3o + pf + cell0, bnew);
+                else
+#pragma unroll
+                    for (int q = 0; q < K; q++) b3o[pf + cell0 + q] = bnew[q];
+            }
+        }
+    }
+    __syncthreads();
+    // b1 = (((b1 - FA_j) + FA_{j+1}) - FB_k) + FB_{k+1}  (A1.c:55, A2.c:23, B1.c:54, B2.c:23)
+    for (int s = w; s < ns; s += nw) {
+        if (l >= L) continue;
+        const long p = (long)(t.j0 + s) * f.sj + t.kb + cell0;
+        T b1[K];
+        if constexpr (VEC) load_cells<T, K>(b1i + p, b1);
+        else
+#pragma unroll
+            for (int q = 0; q < K; q++) b1[q] = b1i[p + q];
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            const long i = cell0 + q;
+            T v = b1[q];
+            v = v - X[s * n + i];
+            v = v + X[(s + 1) * n + i];
+            v = v - X[(ns + 1 + s) * n + i];
+            v = v + X[(2 * ns + 1 + s) * n + i];
+            b1[q] = v;
+        }
+        if constexpr (VEC) store_cells<T, K>(b1o + p, b1);
+        else
+#pragma unroll
+            for (int q = 0; q < K; q++) b1o[p + q] = b1[q];
+    }
+}
+
 }  // namespace mhd
