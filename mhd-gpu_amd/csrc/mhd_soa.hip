@@ -52,6 +52,16 @@ __global__ void __launch_bounds__(64 * W)
                                  reinterpret_cast<T *>(smem_raw));
 }
 
+template <typename T, int K, int W, bool VEC, int CFLM>
+__global__ void __launch_bounds__(64 * W)
+    fluidx_x_kernel(T *__restrict__ u, const T *__restrict__ b, SoaBox bx, const T *__restrict__ dtp, int ntj,
+                    T *__restrict__ cfl_part)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    fluidx_x_tile<T, K, W, VEC, CFLM>(u, b, bx, dtp, xcd_tile(blockIdx.x, gridDim.x), ntj,
+                                      reinterpret_cast<T *>(smem_raw), cfl_part);
+}
+
 template <typename T, int K, int W, bool VEC>
 __global__ void __launch_bounds__(64 * W)
     advect_x_kernel(const T *__restrict__ u, const T *__restrict__ bin, T *__restrict__ bout, SoaBox bx,
@@ -261,6 +271,24 @@ static hipError_t fluidx_soa_k(T *u, const T *b, const SoaBox &bx, const T *dt, 
     const size_t lds = sizeof(T) * fluidx_soa_words<W>(bx.f.ni);
     const dim3 grid(ntj * bx.f.nk), block(64 * W);
     if constexpr (FR == 0) {
+        if (!fast_div_enabled()) {
+            // x frame: direct loads, no LDS staging
+            constexpr bool kvec = K % (16 / sizeof(T)) == 0;
+            const size_t xl = sizeof(T) * fluidx_x_words<W>(bx.f.ni, cflm);
+#define MHD_FX(V, C)                                                                                                 \
+    hipLaunchKernelGGL((fluidx_x_kernel<T, K, W, V, C>), grid, block, xl, st, u, b, bx, dt, ntj, cfl_part)
+            if (kvec && bx.vec_ok) {
+                if (cflm == 1) MHD_FX(kvec, 1);
+                else if (cflm == 2) MHD_FX(kvec, 2);
+                else MHD_FX(kvec, 0);
+            } else {
+                if (cflm == 1) MHD_FX(false, 1);
+                else if (cflm == 2) MHD_FX(false, 2);
+                else MHD_FX(false, 0);
+            }
+#undef MHD_FX
+            return hipGetLastError();
+        }
         if (cflm == 1) {
             hipLaunchKernelGGL((fluidx_soa_kernel<T, K, W, 0, false, 1>), grid, block, lds, st, u, b, bx, dt, ntj,
                                cfl_part);

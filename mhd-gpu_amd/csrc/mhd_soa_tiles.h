@@ -834,4 +834,128 @@ __device__ __forceinline__ void advect_x_tile(const T *__restrict__ u, const T *
     }
 }
 
+// ---------------------------------------------------------------------------
+// fluidx in the x frame with direct loads: a wave's pencil is contiguous, so
+// it reads u, its face fields and the neighbours the face average needs
+// (b2 of pencil j+1, b3 at k+1) straight from global memory, b1 at i+1 by a
+// lane shuffle, and stores u back; no LDS and no barrier.  Same arithmetic
+// as fluidx_soa_tile.  CFLM as there: the u of the tile's pencils is kept
+// in LDS (5 n words per wave) for the CFL speed after a barrier.
+// ---------------------------------------------------------------------------
+template <int W> __host__ __device__ constexpr int fluidx_x_words(int n, int cflm) { return cflm ? 5 * W * n : 0; }
+
+template <typename T, int K, int W, bool VEC, int CFLM>
+__device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__restrict__ b, const SoaBox &bx,
+                                              const T *__restrict__ dtp, int tile, int ntj, T *su,
+                                              T *__restrict__ cfl_part)
+{
+    const Frame &f = bx.f;
+    const TileCtx t = tile_ctx<W>(bx, tile, ntj);
+    const int n = t.n, ns = t.ns, L = n / K;
+    const long N = bx.N;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int l = lane_id(), lc = l % L;
+    const int src_next = lc + 1 == L ? 0 : lc + 1, src_prev = lc == 0 ? L - 1 : lc - 1;
+    const long cell0 = (long)lc * K;
+    const long p = (long)(t.j0 + w) * f.sj + t.kb + cell0;                  // this pencil
+    const long pj = (long)wrapn(t.j0 + w + 1, f.nj) * f.sj + t.kb + cell0;  // pencil j+1
+    const long pk = (long)(t.j0 + w) * f.sj + t.kpb + cell0;                 // plane k+1
+    auto load = [&](const T *src, T (&o)[K]) {
+        if constexpr (VEC) {
+            load_cells<T, K>(src, o);
+        } else {
+#pragma unroll
+            for (int q = 0; q < K; q++) o[q] = src[q];
+        }
+    };
+    if (w < ns) {
+        const T dt = *dtp;
+        T uu[K][5], bxc[K][3];
+        {
+            T tmp[K];
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                load(u + m * N + p, tmp);
+#pragma unroll
+                for (int q = 0; q < K; q++) uu[q][m] = tmp[q];
+            }
+            T b1[K], b1p[K], b2[K], b2p[K], b3[K], b3p[K];
+            load(b + p, b1);
+            load(b + N + p, b2);
+            load(b + 2 * N + p, b3);
+            load(b + N + pj, b2p);
+            load(b + 2 * N + pk, b3p);
+            shift_next<T, K>(b1, b1p, src_next);
+#pragma unroll
+            for (int q = 0; q < K; q++) {
+                // face B to cell centre (fluidx.c:71-81)
+                bxc[q][0] = (T)0.5 * b1[q] + (T)0.5 * b1p[q];
+                bxc[q][1] = (T)0.5 * b2[q] + b2p[q] / (T)2;
+                bxc[q][2] = (T)0.5 * b3[q] + b3p[q] / (T)2;
+            }
+        }
+        tvd1_pencil<T, K, false>(uu, bxc, dt, src_next, src_prev);
+        if (l < L) {
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                T o[K];
+#pragma unroll
+                for (int q = 0; q < K; q++) o[q] = uu[q][m];
+                if constexpr (VEC) {
+                    store_cells<T, K>(u + m * N + p, o);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < K; q++) u[m * N + p + q] = o[q];
+                }
+                if constexpr (CFLM != 0) {
+#pragma unroll
+                    for (int q = 0; q < K; q++) su[(w * 5 + m) * n + cell0 + q] = o[q];
+                }
+            }
+        }
+    }
+    if constexpr (CFLM != 0) {
+        // CFL speed of the new state (calcfl.c:45-56), after a barrier so the
+        // sweep's registers are free: u from LDS, the face fields re-read
+        // (fluidx does not write b); shadow lanes repeat lane l % L
+        __syncthreads();
+        T c = (T)0;
+        if (w < ns) {
+            T b1[K], b2[K], b3[K];
+            load(b + p, b1);
+            load(b + N + p, b2);
+            load(b + 2 * N + p, b3);
+            T b1p[K], b2p[K], b3p[K];
+            if constexpr (CFLM == 2) {
+                load(b + N + pj, b2p);
+                load(b + 2 * N + pk, b3p);
+                shift_next<T, K>(b1, b1p, src_next);
+            }
+#pragma unroll
+            for (int q = 0; q < K; q++) {
+                const T *ui = su + w * 5 * n + cell0 + q;
+                T B;
+                if constexpr (CFLM == 2) {
+                    const T bxa = (T)((double)(b1[q] + b1p[q]) / 2.0);
+                    const T bya = (T)((double)(b2[q] + b2p[q]) / 2.0);
+                    const T bza = (T)((double)(b3[q] + b3p[q]) / 2.0);
+                    B = bxa * bxa + bya * bya + bza * bza;
+                } else {
+                    B = b1[q] * b1[q] + b2[q] * b2[q] + b3[q] * b3[q];
+                }
+                const T sp = cfl_speed<T>(ui[0], ui[n], ui[2 * n], ui[3 * n], ui[4 * n], B);
+                c = c > sp ? c : sp;  // max_2num (calcfl.c:55)
+            }
+        }
+        c = wave_max(c);
+        if (lane_id() == 0) cfl_red_slot<T>()[w] = c;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            T m = cfl_red_slot<T>()[0];
+            for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = fmax(m, cfl_red_slot<T>()[i]);
+            cfl_part[tile] = m;
+        }
+    }
+}
+
 }  // namespace mhd
