@@ -155,6 +155,13 @@ __global__ void __launch_bounds__(64 * kBW)
             advect_x_tile<float, K, kBW, false>(u, bcur, bnext, bd.fr[0], dt + wi.box, wi.tile, bd.ntj[0], smem);
     } else if constexpr (ADV) {
         advect_soa_tile<float, K, kBW, FR>(u, bcur, bnext, bd.fr[FR], dt + wi.box, wi.tile, bd.ntj[FR], smem);
+    } else if constexpr (FR == 0) {
+        // x frame: direct loads, no LDS
+        if (K % 4 == 0 && bd.fr[0].vec_ok)
+            fluidx_x_tile<float, K, kBW, K % 4 == 0, 0>(u, bcur, bd.fr[0], dt + wi.box, wi.tile, bd.ntj[0], smem,
+                                                         nullptr);
+        else
+            fluidx_x_tile<float, K, kBW, false, 0>(u, bcur, bd.fr[0], dt + wi.box, wi.tile, bd.ntj[0], smem, nullptr);
     } else {
         fluidx_soa_tile<float, K, kBW, FR>(u, bcur, bd.fr[FR], dt + wi.box, wi.tile, bd.ntj[FR], smem);
     }
