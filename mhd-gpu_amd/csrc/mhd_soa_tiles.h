@@ -220,46 +220,29 @@ __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__re
             if (d3 >= 0) *reinterpret_cast<V16<T> *>(smem + d3) = v3;
         }
     } else if (FR != 0 && t.vec) {
-        // y/z frames, full tile: items (segment, component, row i); a row of
-        // the tile is one vector spanning the W slots
-        const int eU = 5 * n, eB = eU + 3 * n, eK = eB + n;
-        for (int f0 = tid; f0 < eK; f0 += 4 * nt) {
-            V16<T> v[4];
-            int dstride[4];
-            T *dbase[4];
+        // y/z frames, full tile: a row i of the tile is one vector spanning
+        // the W slots; a thread takes whole rows, issues the row's ten loads
+        // (u 5, b 3, b3 at k+1, b2 of the pencil after the tile) together
+        // and scatters them into the slots
+        for (int i = tid; i < n; i += nt) {
+            const long ro = (long)i * f.si + t.j0;
+            V16<T> vu[5], vb[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int e = f0 + q * nt;
-                dbase[q] = nullptr;
-                if (e < eK) {
-                    const T *src;
-                    if (e < eU) {
-                        const int c = fdiv(e, bx.n_div), i = e - c * n;
-                        src = u + cu<FR>(c) * N + t.kb + (long)i * f.si + t.j0;
-                        dbase[q] = su + c * n + i;
-                        dstride[q] = 5 * n;
-                    } else if (e < eB) {
-                        const int r = e - eU, c = fdiv(r, bx.n_div), i = r - c * n;
-                        src = b + cb<FR>(c) * N + t.kb + (long)i * f.si + t.j0;
-                        dbase[q] = sb + c * n + i;
-                        dstride[q] = 3 * n;
-                    } else {
-                        const int i = e - eB;
-                        src = b + Comp<FR>::b[2] * N + t.kpb + (long)i * f.si + t.j0;
-                        dbase[q] = sbk + i;
-                        dstride[q] = n;
-                    }
-                    v[q] = *reinterpret_cast<const V16<T> *>(src);
-                }
+            for (int c = 0; c < 5; c++) vu[c] = *reinterpret_cast<const V16<T> *>(u + cu<FR>(c) * N + t.kb + ro);
+#pragma unroll
+            for (int c = 0; c < 3; c++) vb[c] = *reinterpret_cast<const V16<T> *>(b + cb<FR>(c) * N + t.kb + ro);
+            vb[3] = *reinterpret_cast<const V16<T> *>(b + Comp<FR>::b[2] * N + t.kpb + ro);
+            const T b2n = b[Comp<FR>::b[1] * N + t.kb + (long)i * f.si + jn];
+#pragma unroll
+            for (int s = 0; s < W; s++) {
+#pragma unroll
+                for (int c = 0; c < 5; c++) su[(s * 5 + c) * n + i] = vget<T>(vu[c], s);
+#pragma unroll
+                for (int c = 0; c < 3; c++) sb[(s * 3 + c) * n + i] = vget<T>(vb[c], s);
+                sbk[s * n + i] = vget<T>(vb[3], s);
             }
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-                if (dbase[q])
-#pragma unroll
-                    for (int s = 0; s < W; s++) dbase[q][s * dstride[q]] = vget<T>(v[q], s);
+            sb2n[i] = b2n;
         }
-        // b2 of the pencil after the tile: one value per row
-        for (int i = tid; i < n; i += nt) sb2n[i] = b[Comp<FR>::b[1] * N + t.kb + (long)i * f.si + jn];
     } else {
         // general tile (partial, or vectors not aligned): one value per item
         const int eU = ns * 5 * n, eB = eU + ns * 3 * n, eN = eB + n, eK = eN + ns * n;
@@ -390,14 +373,18 @@ __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__re
                 *reinterpret_cast<const V16<T> *>(su + (s * 5 + c) * n + i4 * VE);
         }
     } else if (FR != 0 && t.vec) {
-        for (int e = tid; e < 5 * n; e += nt) {
-            const int c = fdiv(e, bx.n_div), i = e - c * n;
-            V16<T> v;
-            if constexpr (sizeof(T) == 4)
-                v = make_float4(su[c * n + i], su[(5 + c) * n + i], su[(10 + c) * n + i], su[(15 + c) * n + i]);
-            else
-                v = make_double2(su[c * n + i], su[(5 + c) * n + i]);
-            *reinterpret_cast<V16<T> *>(u + cu<FR>(c) * N + t.kb + (long)i * f.si + t.j0) = v;
+        // whole rows per thread: gather the W slots of each component
+        for (int i = tid; i < n; i += nt) {
+            const long ro = (long)i * f.si + t.j0 + t.kb;
+#pragma unroll
+            for (int c = 0; c < 5; c++) {
+                V16<T> v;
+                if constexpr (sizeof(T) == 4)
+                    v = make_float4(su[c * n + i], su[(5 + c) * n + i], su[(10 + c) * n + i], su[(15 + c) * n + i]);
+                else
+                    v = make_double2(su[c * n + i], su[(5 + c) * n + i]);
+                *reinterpret_cast<V16<T> *>(u + cu<FR>(c) * N + ro) = v;
+            }
         }
     } else {
         for (int e = tid; e < ns * 5 * n; e += nt) {
