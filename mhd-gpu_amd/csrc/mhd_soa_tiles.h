@@ -373,17 +373,19 @@ __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__re
                 *reinterpret_cast<const V16<T> *>(su + (s * 5 + c) * n + i4 * VE);
         }
     } else if (FR != 0 && t.vec) {
-        // whole rows per thread: gather the W slots of each component
-        for (int i = tid; i < n; i += nt) {
-            const long ro = (long)i * f.si + t.j0 + t.kb;
+        // component by component (the rows of one component leave together,
+        // which lets L2 merge the 16-byte row stores of neighbouring tiles):
+        // gather the W slots of row i
 #pragma unroll
-            for (int c = 0; c < 5; c++) {
+        for (int c = 0; c < 5; c++) {
+            T *uc = u + cu<FR>(c) * N + t.j0 + t.kb;
+            for (int i = tid; i < n; i += nt) {
                 V16<T> v;
                 if constexpr (sizeof(T) == 4)
                     v = make_float4(su[c * n + i], su[(5 + c) * n + i], su[(10 + c) * n + i], su[(15 + c) * n + i]);
                 else
                     v = make_double2(su[c * n + i], su[(5 + c) * n + i]);
-                *reinterpret_cast<V16<T> *>(u + cu<FR>(c) * N + ro) = v;
+                *reinterpret_cast<V16<T> *>(uc + (long)i * f.si) = v;
             }
         }
     } else {
