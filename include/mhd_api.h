@@ -150,6 +150,13 @@ int mhd_ctx_device_state(mhd_ctx *ctx, void **d_u, void **d_b);
 /* layout of each plane: cell (i,j,k) at i + j*sy + k*sz, planes `plane` values
  * apart (sy >= nx, sz >= sy*ny: padded rows/planes avoid power-of-two strides) */
 int mhd_ctx_layout(mhd_ctx *ctx, long *sy, long *sz, long *plane);
+/* kernel form of the y/z advectbyzx sweeps (no counterpart in the
+ * reference; a tuning and test switch): 0 automatic (the default: LDS tiles
+ * for boxes too small to stream, else the fused column stream where
+ * nx <= 1024, else the two-pass stream), 1 LDS tiles, 2 two-pass column
+ * stream, 3 fused column stream (MHD_ERR_ARG when nx > 1024).  Results are
+ * identical in every form. */
+int mhd_ctx_set_advect_form(mhd_ctx *ctx, int form);
 /* time per kernel of one step, for the roofline: runs `reps` steps with
  * events around every launch; ms_out[0..12] = dt, then the 12 sweeps in
  * order (fluidx x, advect x, fluidx y, advect y, fluidx z, advect z,

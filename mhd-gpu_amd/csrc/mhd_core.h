@@ -75,28 +75,40 @@ __device__ __forceinline__ bool tvd1_pencil(T (&uu)[K][5], const T (&bx)[K][3], 
     ok = mhdflux_pencil<T, K, FAST>(u1, bx, v, c) && ok;
 #pragma unroll
     for (int m = 0; m < 5; m++) {
-        T wl[K], fr[K], frn[K], frp[K], wln[K], fl[K], fln[K], flp[K], fx[K], fxp[K];
+        // tvd1.c:59-90 forms dfrp = 0.5*fr[i+1] + (-0.5)*fr[i] and
+        // dfrm = 0.5*fr[i] + (-0.5)*fr[i-1] (likewise for fl).  In IEEE
+        // arithmetic (-0.5)*x == -(0.5*x) and a + (-b) == a - b, so with
+        // hfr = 0.5*fr both are differences of neighbouring hfr, and
+        // dfrm[i] is dfrp[i-1] bit for bit: one product and one difference
+        // per cell instead of four and two.
+        T wl[K], fr[K], wln[K], fl[K], hr[K], hrn[K], hl[K], hln[K], dfrp[K], dfrm[K], dflp[K], dflm[K], fx[K],
+            fxp[K];
 #pragma unroll
         for (int q = 0; q < K; q++) {
             const T wr = u1[q][m] + v[q][m];
             wl[q] = u1[q][m] - v[q][m];
             fr[q] = c * wr;
+            hr[q] = (T)0.5 * fr[q];
         }
-        shift_next<T, K>(fr, frn, src_next);
-        shift_prev<T, K>(fr, frp, src_prev);
         shift_next<T, K>(wl, wln, src_next);
 #pragma unroll
-        for (int q = 0; q < K; q++) fl[q] = c * wln[q];
-        shift_next<T, K>(fl, fln, src_next);
-        shift_prev<T, K>(fl, flp, src_prev);
+        for (int q = 0; q < K; q++) {
+            fl[q] = c * wln[q];
+            hl[q] = (T)0.5 * fl[q];
+        }
+        shift_next<T, K>(hr, hrn, src_next);
+        shift_next<T, K>(hl, hln, src_next);
 #pragma unroll
         for (int q = 0; q < K; q++) {
-            const T dfrp = (T)0.5 * frn[q] + (T)(-0.5) * fr[q];
-            const T dfrm = (T)0.5 * fr[q] + (T)(-0.5) * frp[q];
-            const T dflp = (T)0.5 * fl[q] + (T)(-0.5) * fln[q];
-            const T dflm = (T)0.5 * flp[q] + (T)(-0.5) * fl[q];
-            const T dfr = vanleer<T>(dfrp, dfrm);
-            const T dfl = vanleer<T>(dflp, dflm);
+            dfrp[q] = hrn[q] - hr[q];
+            dflp[q] = hl[q] - hln[q];
+        }
+        shift_prev<T, K>(dfrp, dfrm, src_prev);
+        shift_prev<T, K>(dflp, dflm, src_prev);
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            const T dfr = vanleer<T>(dfrp[q], dfrm[q]);
+            const T dfl = vanleer<T>(dflp[q], dflm[q]);
             fx[q] = (T)0.5 * (fr[q] - fl[q]) + (T)0.5 * (dfr - dfl);
         }
         shift_prev<T, K>(fx, fxp, src_prev);

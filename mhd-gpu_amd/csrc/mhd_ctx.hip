@@ -42,6 +42,7 @@ struct mhd_ctx {
     hipGraph_t graph;
     hipGraphExec_t exec;
     bool captured;
+    int adv_form;              // y/z advectbyzx: 0 automatic, 1 tiles, 2 two-pass stream, 3 fused stream
     hipEvent_t ev0, ev1;
 };
 
@@ -112,9 +113,11 @@ template <typename T> static hipError_t enqueue_sweep(mhd_ctx *c, int s)
         return launch_fluidx_soa<T>((T *)c->u, bcur, fr, c->g, (const T *)c->dt, c->st);
     }
     // x frame, and y/z frames of small boxes: LDS tiles; else column streaming
-    if (fr == 0 || !stream_pays(make_stream_box(fr, c->g)))
+    const bool tiles = c->adv_form == 0 ? !stream_pays(make_stream_box(fr, c->g)) : c->adv_form == 1;
+    if (fr == 0 || tiles)
         return launch_advect_soa<T>((const T *)c->u, bcur, bnext, fr, c->g, (const T *)c->dt, c->st);
-    return launch_advect_stream<T>((const T *)c->u, bcur, bnext, (T *)c->fa, fr, c->g, (const T *)c->dt, c->st);
+    return launch_advect_stream<T>((const T *)c->u, bcur, bnext, (T *)c->fa, fr, c->g, (const T *)c->dt, c->st,
+                                   c->adv_form == 0 ? 0 : c->adv_form - 1);
 }
 
 template <typename T> static hipError_t enqueue_step(mhd_ctx *c)
@@ -285,6 +288,20 @@ int mhd_ctx_get_state(mhd_ctx *c, void *h_u, void *h_b)
 {
     if (!c || !h_u || !h_b) return fail(MHD_ERR_ARG, "mhd_ctx_get_state: null argument");
     return transfer_state(c, h_u, h_b, 0);
+}
+
+int mhd_ctx_set_advect_form(mhd_ctx *c, int form)
+{
+    if (!c) return fail(MHD_ERR_ARG, "mhd_ctx_set_advect_form: null ctx");
+    if (form < 0 || form > 3) return fail(MHD_ERR_ARG, "mhd_ctx_set_advect_form: form must be 0..3");
+    if (form == 3 && (!stream_fusable(make_stream_box(1, c->g)) || !stream_fusable(make_stream_box(2, c->g))))
+        return fail(MHD_ERR_ARG, "mhd_ctx_set_advect_form: the fused stream needs nx <= 1024");
+    const hipError_t e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) return hip_fail(e, "mhd_ctx_set_advect_form");
+    // the sweeps are baked into the captured graph: capture again on the next step
+    drop_graph(c);
+    c->adv_form = form;
+    return MHD_OK;
 }
 
 int mhd_ctx_reset_clock(mhd_ctx *c, double tf)

@@ -49,6 +49,9 @@ inline int stream_waves_b(const StreamBox &sb) { return sb.ntj * ((sb.nk + 1) / 
 // 128^3, 1024 waves, 0.64 ms/step streamed against 0.52 with tiles; 192^3,
 // 3456 waves, 1.48 against 1.55).
 inline bool stream_pays(const StreamBox &sb) { return stream_waves_a(sb) >= 2048; }
+// both passes in one kernel: the pencils of a row fit one workgroup
+inline bool stream_fusable(const StreamBox &sb) { return sb.ntj <= 16; }
+bool stream_fused(const StreamBox &sb);
 
 // cflm (x frame only): 1 / 2 also writes the block maxima of the new state's
 // CFL speed (face / cell-averaged B) to cfl_part, one per tile (the x
@@ -66,11 +69,13 @@ template <typename T> inline int fluidx_tiles_x(const SoaGrid &g)
 template <typename T>
 hipError_t launch_advect_soa(const T *u, const T *bin, T *bout, int fr, const SoaGrid &g, const T *dt,
                              hipStream_t st);
-// advectbyzx of the y (fr 1) or z (fr 2) frame as two column-streaming
-// passes; fa is a scratch plane of N values
+// advectbyzx of the y (fr 1) or z (fr 2) frame as a column stream: form 0
+// fused where stream_fused allows, else two passes; 1 two passes (fa is
+// their scratch plane of N values); 2 fused (hipErrorInvalidValue if the
+// pencils of a row exceed one workgroup)
 template <typename T>
 hipError_t launch_advect_stream(const T *u, const T *bin, T *bout, T *fa, int fr, const SoaGrid &g, const T *dt,
-                                hipStream_t st);
+                                hipStream_t st, int form = 0);
 template <typename T> hipError_t prepare_soa(int nx, int ny, int nz);
 template <typename T>
 hipError_t launch_cfl_soa(const T *u, const T *b, const SoaGrid &g, int avg, unsigned long long *cmax, hipStream_t st);

@@ -94,6 +94,20 @@ __global__ void __launch_bounds__(256) advect_stream_b_kernel(const T *__restric
     if (wave < nwaves) advect_stream_b<T, FR>(u, bin, bout, fa, sb, dtp, wave);
 }
 
+// both passes fused: one workgroup of ntj <= NW waves per plane pair and
+// segment (pencil tiles along j inside the workgroup)
+template <typename T, int FR, int NW>
+__global__ void __launch_bounds__(64 * NW) advect_stream_ab_kernel(const T *__restrict__ u,
+                                                                   const T *__restrict__ bin, T *__restrict__ bout,
+                                                                   StreamBox sb, const T *__restrict__ dtp)
+{
+    __shared__ T xch[64];
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int nk2 = (sb.nk + 1) / 2;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    advect_stream_ab<T, FR>(u, bin, bout, sb, dtp, tile % nk2, tile / nk2, w, xch);
+}
+
 // ---------------------------------------------------------------------------
 // CFL over the SoA state (calcfl.c:13-65): per-cell speed, max -> *cmax bits
 // ---------------------------------------------------------------------------
@@ -373,6 +387,16 @@ hipError_t launch_advect_soa(const T *u, const T *bin, T *bout, int fr, const So
 // Rows per segment: enough segments for about 8192 waves of pass A (8 per
 // SIMD: the streams are latency-bound), but at least 32 rows each (a segment
 // re-reads 8 rows); MHD_STREAM_SEG (rows) overrides.
+// MHD_STREAM_AB=0 keeps the two-pass form where the fused one applies
+bool stream_fused(const StreamBox &sb)
+{
+    static const bool on = [] {
+        const char *v = getenv("MHD_STREAM_AB");
+        return !(v && v[0] == '0');
+    }();
+    return on && stream_fusable(sb);
+}
+
 StreamBox make_stream_box(int fr, const SoaGrid &g)
 {
     StreamBox sb;
@@ -405,10 +429,22 @@ StreamBox make_stream_box(int fr, const SoaGrid &g)
 
 template <typename T>
 hipError_t launch_advect_stream(const T *u, const T *bin, T *bout, T *fa, int fr, const SoaGrid &g, const T *dt,
-                                hipStream_t st)
+                                hipStream_t st, int form)
 {
     if (fr != 1 && fr != 2) return hipErrorInvalidValue;
     const StreamBox sb = make_stream_box(fr, g);
+    if (form == 2 && !stream_fusable(sb)) return hipErrorInvalidValue;
+    if (form == 2 || (form == 0 && stream_fused(sb))) {
+        const dim3 grid(((sb.nk + 1) / 2) * sb.nseg), block(64 * sb.ntj);
+#define SAB(FR, NW) hipLaunchKernelGGL((advect_stream_ab_kernel<T, FR, NW>), grid, block, 0, st, u, bin, bout, sb, dt)
+        if (sb.ntj <= 4) {
+            if (fr == 1) SAB(1, 4); else SAB(2, 4);
+        } else {
+            if (fr == 1) SAB(1, 16); else SAB(2, 16);
+        }
+#undef SAB
+        return hipGetLastError();
+    }
     const int nwa = stream_waves_a(sb), nwb = stream_waves_b(sb);
     const dim3 ga((nwa + 3) / 4), gb((nwb + 3) / 4), block(256);
     if (fr == 1) {
@@ -495,7 +531,7 @@ template <typename T> hipError_t launch_aos_soa(T *soa, T *aos, int ncomp, const
                                              hipStream_t);                                                          \
     template hipError_t prepare_soa<T>(int, int, int);                                                              \
     template hipError_t launch_advect_stream<T>(const T *, const T *, T *, T *, int, const SoaGrid &, const T *,    \
-                                                hipStream_t);                                                       \
+                                                hipStream_t, int);                                                  \
     template hipError_t launch_cfl_soa<T>(const T *, const T *, const SoaGrid &, int, unsigned long long *,        \
                                           hipStream_t);                                                             \
     template hipError_t launch_monitors_soa<T>(const T *, const T *, const SoaGrid &, double *, int, double *,     \

@@ -227,4 +227,133 @@ __device__ __forceinline__ void advect_stream_b(const T *__restrict__ u, const T
     }
 }
 
+// Both passes in one stream (used when the pencils of a row fit one
+// workgroup, nj <= 1024).  A workgroup holds the ntj waves of one plane pair
+// and segment, wave w covering pencils 64w .. 64w+63; a lane streams the
+// pencil j of planes k0 and k0+1 through five tvdb streams: pass A at k0 and
+// k0+1, pass B at the faces k0, k0+1, k0+2.  The b1 update takes FA of pencil
+// j+1 from lane l+1, or, across a wave boundary and for the periodic wrap,
+// from lane 0 of the neighbouring wave through LDS (xch: 2 buffers x 16
+// waves x 2 planes), one barrier per row.  FA never reaches global memory and
+// every operand is read once per plane pair.
+template <typename T, int FR>
+__device__ __forceinline__ void advect_stream_ab(const T *__restrict__ u, const T *__restrict__ bin,
+                                                 T *__restrict__ bout, const StreamBox &sb,
+                                                 const T *__restrict__ dtp, int kq, int seg, int w, T *xch)
+{
+    const int j = w * 64 + (int)lane_id();
+    const bool valid = j < sb.nj;
+    const int jj = valid ? j : sb.nj - 1;
+    const int jm = jj == 0 ? sb.nj - 1 : jj - 1;
+    const int jp = jj + 1 == sb.nj ? 0 : jj + 1;
+    // FA of pencil jp: lane jp & 63 of wave jp >> 6 (lane 0 when another wave)
+    const bool remote = (jp >> 6) != w;
+    const int src = jp & 63;
+    const int wr = jp >> 6;
+    const long N = sb.N;
+    const T *rho = u, *mom = u + Comp<FR>::u[1] * N;
+    const T *b1 = bin + Comp<FR>::b[0] * N, *b2 = bin + Comp<FR>::b[1] * N, *b3 = bin + Comp<FR>::b[2] * N;
+    T *b1o = bout + Comp<FR>::b[0] * N, *b2o = bout + Comp<FR>::b[1] * N, *b3o = bout + Comp<FR>::b[2] * N;
+    const T dt = *dtp;
+    const int s0 = seg * sb.S, s1 = min(s0 + sb.S, sb.ni);
+    const int k0 = 2 * kq;
+    const bool two = k0 + 1 < sb.nk;  // uniform over the workgroup
+    const int km = k0 == 0 ? sb.nk - 1 : k0 - 1;
+    const int k1 = k0 + 1 == sb.nk ? 0 : k0 + 1;
+    const int k2 = k1 + 1 == sb.nk ? 0 : k1 + 1;
+    const long kmb = (long)km * sb.sk, k0b = (long)k0 * sb.sk, k1b = (long)k1 * sb.sk, k2b = (long)k2 * sb.sk;
+    const bool multi = sb.ntj > 1;
+    TvdbStream<T> a0, a1, fb0, fb1, fb2;  // pass A at k0, k1; pass B at the faces k0, k1, k2
+    struct Row {
+        T mm0, m0, rm0, r0, b20;  // pencils j-1, j of plane k0; b2 at k0
+        T mm1, m1, rm1, r1, b21;  // the same at k1
+        T mk, rk, m2, r2;         // pencil j of planes k0-1 and k0+2
+        T b30, b31, b32;          // b3 at k0, k1, k2
+    };
+    auto load_row = [&](int t, Row &q) {
+        const long ri = (long)wrap_row(t, sb.ni) * sb.si;
+        const long a = ri + k0b, c = ri + k1b;
+        q.mm0 = mom[a + jm];
+        q.m0 = mom[a + jj];
+        q.rm0 = rho[a + jm];
+        q.r0 = rho[a + jj];
+        q.b20 = b2[a + jj];
+        q.mk = mom[ri + kmb + jj];
+        q.rk = rho[ri + kmb + jj];
+        q.b30 = b3[a + jj];
+        q.b31 = b3[c + jj];
+        q.m1 = mom[c + jj];
+        q.r1 = rho[c + jj];
+        if (two) {
+            q.mm1 = mom[c + jm];
+            q.rm1 = rho[c + jm];
+            q.b21 = b2[c + jj];
+            q.m2 = mom[ri + k2b + jj];
+            q.r2 = rho[ri + k2b + jj];
+            q.b32 = b3[ri + k2b + jj];
+        } else {
+            q.mm1 = q.rm1 = q.b21 = q.m2 = q.r2 = q.b32 = (T)0;
+        }
+    };
+    Row nx;
+    load_row(s0 - 4, nx);
+    T c10 = 0, c11 = 0;  // b1 of the output row, loaded one row ahead
+    for (int t = s0 - 4; t < s1 + 4; t++) {
+        const Row q = nx;
+        const T d10 = c10, d11 = c11;
+        if (t + 1 < s1 + 4) load_row(t + 1, nx);
+        if (t + 1 >= s0 + 4 && t + 1 < s1 + 4) {
+            const long o = (long)(t - 3) * sb.si + jj;
+            c10 = b1[o + k0b];
+            if (two) c11 = b1[o + k1b];
+        }
+        T b2n0, fa0, b2n1 = 0, fa1 = 0, b3n0, f0, b3n1, f1, f2 = 0;
+        a0.push(q.mm0, q.m0, q.rm0, q.r0, q.b20, dt, b2n0, fa0);
+        fb0.push(q.mk, q.m0, q.rk, q.r0, q.b30, dt, b3n0, f0);
+        fb1.push(q.m0, q.m1, q.r0, q.r1, q.b31, dt, b3n1, f1);
+        if (two) {
+            a1.push(q.mm1, q.m1, q.rm1, q.r1, q.b21, dt, b2n1, fa1);
+            T b3x;
+            fb2.push(q.m1, q.m2, q.r1, q.r2, q.b32, dt, b3x, f2);
+        }
+        if (t >= s0 + 4) {
+            // FA of pencil j+1 at the output row
+            T n0 = shfl_from(fa0, src), n1 = shfl_from(fa1, src);
+            if (multi) {
+                T *x = xch + (t & 1) * 32;
+                if (lane_id() == 0) {
+                    x[w * 2] = fa0;
+                    x[w * 2 + 1] = fa1;
+                }
+                __syncthreads();
+                if (remote) {
+                    n0 = x[wr * 2];
+                    n1 = x[wr * 2 + 1];
+                }
+            }
+            if (valid) {
+                const long o = (long)(t - 4) * sb.si + jj;
+                T v = d10;
+                v = v - fa0;
+                v = v + n0;
+                v = v - f0;
+                v = v + f1;
+                b1o[o + k0b] = v;
+                b2o[o + k0b] = b2n0;
+                b3o[o + k0b] = b3n0;
+                if (two) {
+                    T y = d11;
+                    y = y - fa1;
+                    y = y + n1;
+                    y = y - f1;
+                    y = y + f2;
+                    b1o[o + k1b] = y;
+                    b2o[o + k1b] = b2n1;
+                    b3o[o + k1b] = b3n1;
+                }
+            }
+        }
+    }
+}
+
 }  // namespace mhd

@@ -56,6 +56,7 @@ SIGNATURES = {
     "mhd_ctx_set_state": (_I, [_P, _P, _P]),
     "mhd_ctx_get_state": (_I, [_P, _P, _P]),
     "mhd_ctx_reset_clock": (_I, [_P, ctypes.c_double]),
+    "mhd_ctx_set_advect_form": (_I, [_P, _I]),
     "mhd_ctx_step": (_I, [_P, _I]),
     "mhd_ctx_step_timed": (_I, [_P, _I, _FP]),
     "mhd_ctx_sync": (_I, [_P]),
@@ -250,6 +251,12 @@ class Simulation:
 
     def reset_clock(self, tf=math.inf):
         check(lib().mhd_ctx_reset_clock(self.ctx, float(tf)))
+
+    # kernel form of the y/z advectbyzx (identical results in every form)
+    ADVECT_FORMS = {"auto": 0, "tiles": 1, "two_pass": 2, "fused": 3}
+
+    def set_advect_form(self, form="auto"):
+        check(lib().mhd_ctx_set_advect_form(self.ctx, self.ADVECT_FORMS[form]))
 
     def step(self, nsteps=1):
         check(lib().mhd_ctx_step(self.ctx, int(nsteps)))

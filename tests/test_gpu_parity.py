@@ -137,7 +137,7 @@ def test_config1_alfven_32cube_200_steps(M, oracle):
                                                      ((42, 22, 26), 504, 0.45, 3, math.inf),
                                                      # pencils of 2 and 3 cells; a partial 64-pencil tile
                                                      ((70, 2, 3), 505, 0.4, 3, math.inf),
-                                                     # y/z pencils cut into segments, the last one short; an odd plane count
+                                                     # an odd plane count
                                                      ((64, 100, 45), 506, 0.35, 2, math.inf)])
 def test_ctx_steps_match_oracle(M, oracle, shape, seed, amp, steps, tf):
     nx, ny, nz = shape
@@ -154,6 +154,36 @@ def test_ctx_steps_match_oracle(M, oracle, shape, seed, amp, steps, tf):
     t, t64 = sim.time()
     assert abs(t64 - 2 * do.astype(np.float64).sum()) < 1e-12
     sim.close()
+
+
+_FORM_ORACLE = {}
+
+
+# Every kernel form of the y/z advectbyzx, forced at small sizes (the automatic
+# choice streams only boxes of >= 2048 pencil segments): one workgroup wave
+# (nx 64, 70), three and five waves with a partial last one (nx 130, 300: the
+# fused stream's 4- and 16-wave builds), odd plane counts, pencils of 2 cells,
+# and pencils cut into segments with a short last one (ny 100).
+@pytest.mark.parametrize("form", ["tiles", "two_pass", "fused"])
+@pytest.mark.parametrize("shape", [(64, 100, 45), (130, 40, 33), (300, 36, 20), (70, 2, 3)])
+def test_advect_forms_match_oracle(M, oracle, shape, form):
+    nx, ny, nz = shape
+    print("advect form", form, shape, flush=True)
+    u0, b0 = random_state(nx, ny, nz, seed=600 + nx, amp=0.4)
+    if shape not in _FORM_ORACLE:
+        uo, bo = u0.copy(), b0.copy()
+        do = oracle.run(uo, bo, nx, ny, nz, 2)
+        _FORM_ORACLE[shape] = (uo, bo, do)
+    uo, bo, do = _FORM_ORACLE[shape]
+    sim = M.Simulation(nx, ny, nz)
+    sim.set_advect_form(form)
+    sim.set_state(u0, b0)
+    sim.step(2)
+    ug, bg = sim.get_state()
+    dg = sim.dts()
+    sim.close()
+    assert bitwise(dg, do)
+    assert same(ug, uo) and same(bg, bo)
 
 
 @pytest.mark.parametrize("avg", [False, True])
