@@ -36,8 +36,7 @@ template <typename T> struct TvdbStream {
     T vr2 = 0, vr1 = 0;                  // raw face velocity at t-2, t-1
     T vg2 = 0;                           // smoothed velocity at t-2
     T bw4 = 0, bw3 = 0, bw2 = 0, bw1 = 0;  // field at t-4 .. t-1
-    T bv2 = 0;                           // b*vg at t-2
-    T vh4 = 0, vh3 = 0;                  // vh at t-4, t-3
+    bool up4 = false, up3 = false;       // vh > 0 at t-4, t-3 (vh is only ever tested for its sign)
     T f13 = 0;                           // half-step flux at t-3
     T g5 = 0, g4 = 0, g3 = 0;            // vg*b1 at t-5 .. t-3
     T fl5 = 0;                           // flux at t-5
@@ -48,16 +47,18 @@ template <typename T> struct TvdbStream {
         // A1.c:41 vx = (vx[i-1] + vx[i+1] + 2.0*vx)/4.0 at i = t-1
         const T vg1 = (T)(((double)(vr2 + vr0) + 2.0 * (double)vr1) / 4.0);
         // tvdb.c:27 vh = 0.5*vg + 0.5*vg[i+1] at i = t-2; tvdb.c:33 b*vg at t-1
-        const T vh2 = (T)0.5 * vg2 + (T)0.5 * vg1;
+        const bool up2 = (T)0.5 * vg2 + (T)0.5 * vg1 > (T)0;
         const T bv1 = bw1 * vg1;
-        // tvdb.c:36-46 upwind half-step flux at t-2
-        const T f12 = vh2 > (T)0 ? bv2 : bv1;
+        // tvdb.c:36-46 upwind half-step flux at t-2 (b*vg at t-2 formed
+        // again from the stored field and velocity: the same product)
+        const T bv2 = bw2 * vg2;
+        const T f12 = up2 ? bv2 : bv1;
         // tvdb.c:51 b1 = b - (flux1 - flux1[i-1])*dt/2.0 at t-2
         const T b12 = (T)((double)bw2 - (double)((f12 - f13) * dt) / 2.0);
         const T g2 = vg2 * b12;
         // tvdb.c:54-78 limited flux at i = t-4 (g[i-1..i+2] = g5, g4, g3, g2)
         T w, wp, wm;
-        if (vh4 > (T)0) {
+        if (up4) {
             w = g4;
             wp = (T)((double)(g3 - w) / 2.0);
             wm = (T)((double)(w - g5) / 2.0);
@@ -78,9 +79,8 @@ template <typename T> struct TvdbStream {
         bw3 = bw2;
         bw2 = bw1;
         bw1 = bt;
-        bv2 = bv1;
-        vh4 = vh3;
-        vh3 = vh2;
+        up4 = up3;
+        up3 = up2;
         f13 = f12;
         g5 = g4;
         g4 = g3;
