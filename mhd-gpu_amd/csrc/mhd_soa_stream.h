@@ -284,76 +284,88 @@ __device__ __forceinline__ void advect_stream_ab(const T *__restrict__ u, const 
         q.b31 = b3[c + jj];
         q.m1 = mom[c + jj];
         q.r1 = rho[c + jj];
-        if (two) {
-            q.mm1 = mom[c + jm];
-            q.rm1 = rho[c + jm];
-            q.b21 = b2[c + jj];
-            q.m2 = mom[ri + k2b + jj];
-            q.r2 = rho[ri + k2b + jj];
-            q.b32 = b3[ri + k2b + jj];
-        } else {
-            q.mm1 = q.rm1 = q.b21 = q.m2 = q.r2 = q.b32 = (T)0;
-        }
+        q.mm1 = mom[c + jm];
+        q.rm1 = rho[c + jm];
+        q.b21 = b2[c + jj];
+        q.m2 = mom[ri + k2b + jj];
+        q.r2 = rho[ri + k2b + jj];
+        q.b32 = b3[ri + k2b + jj];
     };
     Row nx;
     load_row(s0 - 4, nx);
-    T c10 = 0, c11 = 0;  // b1 of the output row, loaded one row ahead
-    for (int t = s0 - 4; t < s1 + 4; t++) {
-        const Row q = nx;
-        const T d10 = c10, d11 = c11;
-        if (t + 1 < s1 + 4) load_row(t + 1, nx);
-        if (t + 1 >= s0 + 4 && t + 1 < s1 + 4) {
-            const long o = (long)(t - 3) * sb.si + jj;
-            c10 = b1[o + k0b];
-            if (two) c11 = b1[o + k1b];
-        }
-        T b2n0, fa0, b2n1 = 0, fa1 = 0, b3n0, f0, b3n1, f1, f2 = 0;
+    // the five pushes of row t; both pass-A streams and all three face
+    // streams run also for the last plane of an odd count (two false),
+    // whose second-plane results are dropped, so that the row is straight
+    // line code
+    T fa0, fa1, f0, f1, f2, b2n0, b2n1, b3n0, b3n1;
+    auto push_row = [&](const Row &q) {
+        T b3x;
         a0.push(q.mm0, q.m0, q.rm0, q.r0, q.b20, dt, b2n0, fa0);
+        a1.push(q.mm1, q.m1, q.rm1, q.r1, q.b21, dt, b2n1, fa1);
         fb0.push(q.mk, q.m0, q.rk, q.r0, q.b30, dt, b3n0, f0);
         fb1.push(q.m0, q.m1, q.r0, q.r1, q.b31, dt, b3n1, f1);
-        if (two) {
-            a1.push(q.mm1, q.m1, q.rm1, q.r1, q.b21, dt, b2n1, fa1);
-            T b3x;
-            fb2.push(q.m1, q.m2, q.r1, q.r2, q.b32, dt, b3x, f2);
-        }
-        if (t >= s0 + 4) {
-            // FA of pencil j+1 at the output row
-            T n0 = shfl_from(fa0, src), n1 = shfl_from(fa1, src);
-            if (multi) {
-                T *x = xch + (t & 1) * 32;
-                if (lane_id() == 0) {
-                    x[w * 2] = fa0;
-                    x[w * 2 + 1] = fa1;
-                }
-                __syncthreads();
-                if (remote) {
-                    n0 = x[wr * 2];
-                    n1 = x[wr * 2 + 1];
-                }
-            }
-            if (valid) {
-                const long o = (long)(t - 4) * sb.si + jj;
-                T v = d10;
-                v = v - fa0;
-                v = v + n0;
-                v = v - f0;
-                v = v + f1;
-                b1o[o + k0b] = v;
-                b2o[o + k0b] = b2n0;
-                b3o[o + k0b] = b3n0;
-                if (two) {
-                    T y = d11;
-                    y = y - fa1;
-                    y = y + n1;
-                    y = y - f1;
-                    y = y + f2;
-                    b1o[o + k1b] = y;
-                    b2o[o + k1b] = b2n1;
-                    b3o[o + k1b] = b3n1;
-                }
-            }
-        }
+        fb2.push(q.m1, q.m2, q.r1, q.r2, q.b32, dt, b3x, f2);
+    };
+    // the first eight rows fill the windows
+    for (int t = s0 - 4; t < s0 + 4; t++) {
+        const Row q = nx;
+        load_row(t + 1, nx);
+        push_row(q);
     }
+    // b1 of the output row, loaded one row ahead (rows wrap, so the load
+    // after the last row stays inside the plane)
+    auto load_b1 = [&](int r, T &c0, T &c1) {
+        const long o = (long)wrap_row(r, sb.ni) * sb.si + jj;
+        c0 = b1[o + k0b];
+        c1 = b1[o + k1b];
+    };
+    T c10, c11;
+    load_b1(s0, c10, c11);
+    // row t >= s0 + 4 emits row t - 4; the operands of row t + 1 and b1 of
+    // row t - 3 are loaded while row t is computed
+    auto row = [&](int t) {
+        const Row q = nx;
+        const T d10 = c10, d11 = c11;
+        load_row(t + 1, nx);
+        load_b1(t - 3, c10, c11);
+        push_row(q);
+        // FA of pencil j+1 at the output row
+        T n0 = shfl_from(fa0, src), n1 = shfl_from(fa1, src);
+        if (multi) {
+            T *x = xch + (t & 1) * 32;
+            if (lane_id() == 0) {
+                x[w * 2] = fa0;
+                x[w * 2 + 1] = fa1;
+            }
+            __syncthreads();
+            if (remote) {
+                n0 = x[wr * 2];
+                n1 = x[wr * 2 + 1];
+            }
+        }
+        if (valid) {
+            const long o = (long)(t - 4) * sb.si + jj;
+            T v = d10;
+            v = v - fa0;
+            v = v + n0;
+            v = v - f0;
+            v = v + f1;
+            b1o[o + k0b] = v;
+            b2o[o + k0b] = b2n0;
+            b3o[o + k0b] = b3n0;
+            if (two) {
+                T y = d11;
+                y = y - fa1;
+                y = y + n1;
+                y = y - f1;
+                y = y + f2;
+                b1o[o + k1b] = y;
+                b2o[o + k1b] = b2n1;
+                b3o[o + k1b] = b3n1;
+            }
+        }
+    };
+    for (int t = s0 + 4; t < s1 + 4; t++) row(t);
 }
 
 }  // namespace mhd
