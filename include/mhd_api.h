@@ -48,6 +48,11 @@ int mhd_device_sync(void);
 int mhd_dev_copy_kernel(void *dst, const void *src, size_t bytes);
 /* free and total device memory in bytes */
 int mhd_mem_info(size_t *free_bytes, size_t *total_bytes);
+/* hardware property the FP32 limiter relies on (no counterpart in the
+ * reference): *rcp_f64_err = max |1 - D*rcp(D)| of the FP64 reciprocal
+ * instruction over every finite nonzero float-valued D (exhaustive scan,
+ * a few milliseconds); the limiter's exactness needs it below 2^-22 */
+int mhd_selfcheck(double *rcp_f64_err);
 
 /* ---- step-level drop-ins (device pointers, FP32) ------------------------
  * Each replaces one wrapper of HIP/hip_subroutine.h:4-12 (CUDA originals in

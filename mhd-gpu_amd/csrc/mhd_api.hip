@@ -156,6 +156,22 @@ int mhd_dev_copy_kernel(void *dst, const void *src, size_t bytes)
     return sync_check(launch_copy(dst, src, bytes, 0), "mhd_dev_copy_kernel");
 }
 
+int mhd_selfcheck(double *rcp_f64_err)
+{
+    if (!rcp_f64_err) return fail(MHD_ERR_ARG, "mhd_selfcheck: null argument");
+    unsigned long long *d = nullptr;
+    hipError_t e = hipMalloc(&d, sizeof *d);
+    if (e != hipSuccess) return hip_fail(e, "mhd_selfcheck");
+    unsigned long long h = 0;
+    e = hipMemset(d, 0, sizeof *d);
+    if (e == hipSuccess) e = launch_rcp_scan(d, 0);
+    if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof h, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "mhd_selfcheck");
+    memcpy(rcp_f64_err, &h, sizeof h);
+    return MHD_OK;
+}
+
 int mhd_device_sync(void)
 {
     hipError_t e = hipDeviceSynchronize();

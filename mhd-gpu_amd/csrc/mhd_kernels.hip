@@ -470,6 +470,29 @@ hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st)
     return hipGetLastError();
 }
 
+// max |1 - D*rcp(D)| of v_rcp_f64 over every finite nonzero float-valued
+// D, as FP64 bits (non-negative doubles order like their bit patterns); the
+// bound the FP32 limiter quotient (limiter_quotient) rests on
+__global__ void __launch_bounds__(256) rcp_scan_kernel(unsigned long long *maxbits)
+{
+    const unsigned long long nt = (unsigned long long)gridDim.x * blockDim.x;
+    double m = 0.0;
+    for (unsigned long long b = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; b < (1ull << 32); b += nt) {
+        const float s = __uint_as_float((unsigned)b);
+        if (!__builtin_isfinite(s) || s == 0.0f) continue;
+        const double D = (double)s;
+        const double e = fabs(fma(-D, __builtin_amdgcn_rcp(D), 1.0));
+        m = e > m ? e : m;
+    }
+    atomicMax(maxbits, (unsigned long long)__double_as_longlong(m));
+}
+
+hipError_t launch_rcp_scan(unsigned long long *maxbits, hipStream_t st)
+{
+    hipLaunchKernelGGL(rcp_scan_kernel, dim3(4096), dim3(256), 0, st, maxbits);
+    return hipGetLastError();
+}
+
 __global__ void zero_u64_kernel(unsigned long long *p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;

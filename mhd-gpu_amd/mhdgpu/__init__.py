@@ -43,6 +43,7 @@ SIGNATURES = {
     "mhd_device_sync": (_I, []),
     "mhd_dev_copy_kernel": (_I, [_P, _P, _SZ]),
     "mhd_mem_info": (_I, [_SZP, _SZP]),
+    "mhd_selfcheck": (_I, [ctypes.POINTER(ctypes.c_double)]),
     "mhd_fluidx": (_I, [_P, _P, _IP, _IP, _IP, _P]),
     "mhd_advectbyzx": (_I, [_P, _P, _IP, _IP, _IP, _P]),
     "mhd_transpose12": (_I, [_P, _P, _P, _P, _IP, _IP, _IP]),
@@ -223,6 +224,14 @@ def mem_info():
     f, t = ctypes.c_size_t(), ctypes.c_size_t()
     check(lib().mhd_mem_info(ctypes.byref(f), ctypes.byref(t)))
     return f.value, t.value
+
+
+def selfcheck():
+    """max |1 - D*rcp(D)| of the FP64 reciprocal instruction over every
+    float-valued D: the bound the FP32 limiter quotient relies on."""
+    e = ctypes.c_double()
+    check(lib().mhd_selfcheck(ctypes.byref(e)))
+    return e.value
 
 
 class Simulation:

@@ -37,6 +37,15 @@ def gpu_op(M, op, u0, b0, nx, ny, nz, dt):
     return du.to_host(), db.to_host()
 
 
+def test_rcp_f64_bound_of_the_limiter_quotient(M):
+    # the FP32 limiter forms 2pm/(p+m) from the FP64 reciprocal instruction
+    # and one Newton step; its equality with the IEEE quotient needs the
+    # instruction within 2^-22 of 1/D on every float-valued D (exhaustive)
+    err = M.selfcheck()
+    print("rcp_f64 max relative error 2^%.2f" % math.log2(err), flush=True)
+    assert 0.0 < err < 2.0 ** -22
+
+
 SHAPES = [(16, 4, 3), (48, 5, 4), (64, 3, 3), (96, 3, 5), (128, 2, 3), (192, 2, 2), (256, 3, 2), (320, 1, 2),
           (384, 2, 1), (512, 1, 2), (24, 1, 1), (40, 7, 1)]
 

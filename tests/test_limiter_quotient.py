@@ -1,0 +1,84 @@
+"""The FP32 limiter quotient of the sweeps (2pm/(p+m): an FP64 reciprocal
+estimate, one Newton step, one remainder correction, then rounding to FP32)
+against the IEEE double division the reference performs (tvd1.c:71-81,
+tvdb.c:72-76).  The corrected double quotient itself equals the IEEE one
+(N/D is never within 2^-78 of a double rounding midpoint: N has 48
+significant bits, D 24), which a dropped correction step breaks at once.  The reciprocal estimate is modelled with a planted relative
+error of up to 2^-24 (the instruction's measured bound is 2^-24.39; the GPU
+test test_rcp_f64_bound_of_the_limiter_quotient rechecks it on the device),
+including both extremes; fma is evaluated exactly with fractions.
+"""
+from fractions import Fraction
+
+import numpy as np
+
+
+def fma(a, b, c):
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def ieee(p, m):
+    p, m = np.float32(p), np.float32(m)
+    if not np.float32(p * m) > 0:
+        return np.float32(0)
+    s = np.float32(p + m)
+    return np.float32(2.0 * float(p) * float(m) / float(s))
+
+
+def fast_double(p, m, delta):
+    """the double quotient before the final rounding to FP32"""
+    s = np.float32(p + m)
+    N, D = 2.0 * float(p) * float(m), float(s)
+    y = float(Fraction(1) / Fraction(D) * (1 + delta))  # planted estimate error
+    e = fma(-D, y, 1.0)
+    y = fma(y, e, y)
+    q = N * y
+    r = fma(-D, q, N)
+    return fma(r, y, q)
+
+
+def fast(p, m, delta):
+    p, m = np.float32(p), np.float32(m)
+    if not np.float32(p * m) > 0:
+        return np.float32(0)
+    return np.float32(fast_double(p, m, delta))
+
+
+def test_limiter_quotient_equals_ieee_with_planted_reciprocal_error():
+    rng = np.random.default_rng(11)
+    n = 6000
+    # float bit patterns: random exponents across the range, plus pairs near
+    # each other (sums close to 2p) and pairs with equal exponents
+    pb = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+    mb = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+    mb[::3] = (mb[::3] & np.uint32(0x807FFFFF)) | (pb[::3] & np.uint32(0x7F800000))
+    mb[1::3] = (pb[1::3] + rng.integers(-40, 40, len(mb[1::3])).astype(np.uint32)) & np.uint32(0xFFFFFFFF)
+    p = pb.view(np.float32)
+    m = mb.view(np.float32)
+    eps = Fraction(1, 1 << 24)
+    deltas = [eps, -eps, Fraction(0)]
+    checked = 0
+    with np.errstate(all="ignore"):
+        for i in range(n):
+            if not (np.isfinite(p[i]) and np.isfinite(m[i])) or not np.isfinite(np.float32(p[i] + m[i])):
+                continue
+            want = ieee(p[i], m[i])
+            d = deltas[i % 3] if i % 4 else Fraction(int(rng.integers(-(1 << 20), 1 << 20)), 1 << 44)
+            got = fast(p[i], m[i], d)
+            assert want.view(np.uint32) == got.view(np.uint32), (p[i], m[i], d)
+            if np.float32(p[i] * m[i]) > 0:
+                s32 = np.float32(p[i] + m[i])
+                assert fast_double(p[i], m[i], d) == 2.0 * float(p[i]) * float(m[i]) / float(s32)
+                checked += 1
+    assert checked > n // 4
+
+
+def test_limiter_quotient_exact_quotients():
+    # quotients that are exact doubles
+    cases = [(1.0, 1.0), (3.0, 1.0), (0.75, 0.25), (2.0 ** -140, 2.0 ** -140), (1.5, 0.5), (6.0, 2.0),
+             (float(np.float32(1 + 2 ** -23)), 1.0), (2.0 ** 100, 2.0 ** 100), (-5.0, -3.0)]
+    eps = Fraction(1, 1 << 24)
+    with np.errstate(all="ignore"):
+        for p, m in cases:
+            for d in (eps, -eps, Fraction(0)):
+                assert ieee(p, m).view(np.uint32) == fast(p, m, d).view(np.uint32), (p, m, d)
