@@ -106,6 +106,33 @@ __device__ __forceinline__ float limiter_quotient(float p, float m)
     return (float)q;
 }
 
+// IEEE FP64 square root for x = 0, +inf or x >= 2^-767: the compiler's own
+// sequence (reciprocal square root estimate, a coupled Newton step on the
+// root and the half reciprocal, two residual corrections; zero and +inf pass
+// through) without its rescaling of arguments below 2^-767.  The sweeps
+// take roots of float-valued doubles and of quotients of floats, never that
+// small.  Both sequences round correctly, so they agree on every argument.
+__device__ __forceinline__ double sqrt_ranged(double x)
+{
+    const double r = __builtin_amdgcn_rsq(x);
+    double g = x * r, h = r * 0.5;
+    const double e = fma(-h, g, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    g = fma(d, h, g);
+    return (x == 0.0 || x == __builtin_inf()) ? x : g;
+}
+template <typename T> __device__ __forceinline__ double sqrt_of(double x)
+{
+    if constexpr (sizeof(T) == 4)
+        return sqrt_ranged(x);
+    else
+        return sqrt(x);
+}
+
 // van Leer harmonic mean: the test is a product in T, the value
 // 2*p*m/(p+m) in double with p+m summed in T (tvd1.c:71-81, tvdb.c:72-76).
 template <typename T> __device__ __forceinline__ T vanleer(T p, T m)
@@ -145,7 +172,7 @@ __device__ __forceinline__ void mhdflux_cell(const T *u, const T *b, T *v, T &ci
     v[4] = (u5 + ps) * vx - b1 * (b1 * u2 + b2 * u3 + b3 * u4) / u1;
     const T p = (T)((double)ps - (double)B / 2.0);
     const T g = gamma_c<T>();
-    ci = (T)(fabs((double)vx) + sqrt(fabs((double)((B + g * p) / u1))));
+    ci = (T)(fabs((double)vx) + sqrt_of<T>(fabs((double)((B + g * p) / u1))));
 }
 
 // ---- exact fast FP32 division ------------------------------------------
@@ -195,7 +222,7 @@ __device__ __forceinline__ void mhdflux_cell_fast(const float *u, const float *b
     v[3] = u4 * vx - b3 * b1;
     v[4] = (u5 + ps) * vx - div_by(b1 * (b1 * u2 + b2 * u3 + b3 * u4), r1, ok);
     const float p = (float)((double)ps - (double)B / 2.0);
-    ci = (float)(fabs((double)vx) + sqrt(fabs((double)div_by(B + g * p, r1, ok))));
+    ci = (float)(fabs((double)vx) + sqrt_ranged(fabs((double)div_by(B + g * p, r1, ok))));
 }
 
 // Per-cell fast speed of calcfl.c:45-54 (face B at the cell, or the cell
@@ -204,12 +231,16 @@ template <typename T>
 __device__ __forceinline__ T cfl_speed(T u1, T u2, T u3, T u4, T u5, T B)
 {
     const T g = gamma_c<T>();
-    const T t1 = (T)fabs((double)(u2 / u1)), t2 = (T)fabs((double)(u3 / u1)), t3 = (T)fabs((double)(u4 / u1));
-    const T v = t1 > t2 ? (t1 > t3 ? t1 : t3) : (t2 > t3 ? t2 : t3);
+    // calcfl.c:45-49 takes the largest of |u2/u1|, |u3/u1|, |u4/u1|.  A
+    // correctly rounded division is monotonic in the dividend, so that is the
+    // quotient of the largest |u_k| (picked by the same comparisons, which
+    // keeps the choice among NaNs too): one division instead of three.
+    const T a1 = fabs(u2), a2 = fabs(u3), a3 = fabs(u4);
+    const T v = (a1 > a2 ? (a1 > a3 ? a1 : a3) : (a2 > a3 ? a2 : a3)) / fabs(u1);
     const T S = u2 * u2 + u3 * u3 + u4 * u4;
     const T ps = pstar<T>(u1, u5, S, B);
     const T p = (T)((double)ps - (double)B / 2.0);
-    return (T)((double)v + sqrt(fabs((double)B * 2.0 + (double)(g * p)) / (double)u1));
+    return (T)((double)v + sqrt_of<T>(fabs((double)B * 2.0 + (double)(g * p)) / (double)u1));
 }
 
 // ---- wave-level helpers: a pencil of n = L*K cells lives in one wave, lane
