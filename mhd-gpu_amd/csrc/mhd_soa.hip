@@ -267,6 +267,10 @@ template <typename T, int K, int FR> static hipError_t soa_attr()
                 a = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
                 if (a != hipSuccess) return a;
             }
+        } else {
+            a = hipFuncSetAttribute((const void *)fluidx_soa_kernel<T, K, 2 * W, FR, false>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            if (a != hipSuccess) return a;
         }
         return hipFuncSetAttribute((const void *)advect_soa_kernel<T, K, W, FR>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
@@ -311,6 +315,18 @@ static hipError_t fluidx_soa_k(T *u, const T *b, const SoaBox &bx, const T *dt, 
         if (cflm == 2) {
             hipLaunchKernelGGL((fluidx_soa_kernel<T, K, W, 0, false, 2>), grid, block, lds, st, u, b, bx, dt, ntj,
                                cfl_part);
+            return hipGetLastError();
+        }
+    }
+    if constexpr (FR != 0) {
+        // y/z frames: tiles two vectors wide (32-byte rows: half the L2
+        // requests of 16-byte rows) where two such tiles fit a CU's LDS
+        constexpr int W2 = 2 * W;
+        const size_t lds2 = sizeof(T) * fluidx_soa_words<W2>(bx.f.ni);
+        if (!fast_div_enabled() && lds2 <= 80 * 1024) {
+            const int ntj2 = (bx.f.nj + W2 - 1) / W2;
+            hipLaunchKernelGGL((fluidx_soa_kernel<T, K, W2, FR, false>), dim3(ntj2 * bx.f.nk), dim3(64 * W2), lds2, st,
+                               u, b, bx, dt, ntj2, (T *)nullptr);
             return hipGetLastError();
         }
     }

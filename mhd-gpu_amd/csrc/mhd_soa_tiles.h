@@ -137,7 +137,8 @@ struct TileCtx {
     bool vec;                  // the 16-byte vector path applies
 };
 
-template <int W> __device__ __forceinline__ TileCtx tile_ctx(const SoaBox &bx, int tile, int ntj)
+// VEW: the vector width in cells (a y/z tile row is whole vectors)
+template <int W, int VEW = W> __device__ __forceinline__ TileCtx tile_ctx(const SoaBox &bx, int tile, int ntj)
 {
     TileCtx t;
     const Frame &f = bx.f;
@@ -151,8 +152,8 @@ template <int W> __device__ __forceinline__ TileCtx tile_ctx(const SoaBox &bx, i
     t.kpb = (long)t.kp * f.sk;
     t.kmb = (long)t.km * f.sk;
     // x frame: pencils of n values, n a multiple of the vector width; y/z
-    // frames: a full tile row of W cells starting at a vector boundary
-    t.vec = bx.vec_ok && (f.si == 1 || t.ns == W);
+    // frames: tile rows of whole vectors starting at a vector boundary
+    t.vec = bx.vec_ok && (f.si == 1 || t.ns % VEW == 0);
     return t;
 }
 
@@ -162,6 +163,43 @@ template <int W> __device__ __forceinline__ TileCtx tile_ctx(const SoaBox &bx, i
 //      sbk[W][n] (b3 of the tile's pencils at k+1)
 // ---------------------------------------------------------------------------
 template <int W> __host__ __device__ constexpr int fluidx_soa_words(int n) { return (9 * W + 1) * n; }
+
+// K consecutive cells from / to p, as 16-byte vectors when K allows (p
+// then 16-byte aligned)
+template <typename T, int K>
+__device__ __forceinline__ void load_cells(const T *__restrict__ p, T (&o)[K])
+{
+    constexpr int VE = Vec16<T>::n;
+    if constexpr (K % VE == 0) {
+#pragma unroll
+        for (int v = 0; v < K / VE; v++) {
+            const V16<T> x = *reinterpret_cast<const V16<T> *>(p + v * VE);
+#pragma unroll
+            for (int e = 0; e < VE; e++) o[v * VE + e] = vget<T>(x, e);
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < K; q++) o[q] = p[q];
+    }
+}
+
+template <typename T, int K>
+__device__ __forceinline__ void store_cells(T *__restrict__ p, const T (&o)[K])
+{
+    constexpr int VE = Vec16<T>::n;
+    if constexpr (K % VE == 0 && sizeof(T) == 4) {
+#pragma unroll
+        for (int v = 0; v < K / VE; v++)
+            *reinterpret_cast<float4 *>(p + v * VE) = make_float4(o[v * VE], o[v * VE + 1], o[v * VE + 2], o[v * VE + 3]);
+    } else if constexpr (K % VE == 0) {
+#pragma unroll
+        for (int v = 0; v < K / VE; v++)
+            *reinterpret_cast<double2 *>(p + v * VE) = make_double2(o[v * VE], o[v * VE + 1]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < K; q++) p[q] = o[q];
+    }
+}
 
 // CFLM (x frame only): 0 = plain sweep; 1 / 2 = also the CFL speed of the
 // new state (calcfl.c:45-56, face B / cell-averaged B of OpenMP/mhd.f90:
@@ -174,9 +212,9 @@ __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__re
 {
     static_assert(CFLM == 0 || FR == 0, "the CFL is fused into the x sweep only");
     constexpr int VE = Vec16<T>::n;
-    static_assert(W == VE, "tile width is one 16-byte vector");
+    static_assert(W == VE || (FR != 0 && W == 2 * VE), "tile width is one 16-byte vector (two in the y/z frames)");
     const Frame &f = bx.f;
-    const TileCtx t = tile_ctx<W>(bx, tile, ntj);
+    const TileCtx t = tile_ctx<W, VE>(bx, tile, ntj);
     const int n = t.n, ns = t.ns, L = n / K;
     const long N = bx.N;
     T *su = smem, *sb = su + 5 * W * n, *sb2n = sb + 3 * W * n, *sbk = sb2n + n;
@@ -220,28 +258,35 @@ __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__re
             if (d3 >= 0) *reinterpret_cast<V16<T> *>(smem + d3) = v3;
         }
     } else if (FR != 0 && t.vec) {
-        // y/z frames, full tile: a row i of the tile is one vector spanning
-        // the W slots; a thread takes whole rows, issues the row's ten loads
-        // (u 5, b 3, b3 at k+1, b2 of the pencil after the tile) together
-        // and scatters them into the slots
-        for (int i = tid; i < n; i += nt) {
-            const long ro = (long)i * f.si + t.j0;
+        // y/z frames: a row i of the tile is H = ns / VE vectors of VE slots
+        // (H = 1 or 2).  A thread takes one vector of a row, adjacent threads
+        // the adjacent vectors (so that the loads of a row merge into one
+        // request), issues its loads (u 5, b 3, b3 at k+1; b2 of the pencil
+        // after the tile with the last vector) together and scatters them
+        // into the slots
+        const int H = ns / VE;
+        for (int e = tid; e < n * H; e += nt) {
+            const int i = H == 2 ? e >> 1 : e, h = e - i * H;
+            const long ro = (long)i * f.si + t.j0 + h * VE;
+            const bool last = h == H - 1;
             V16<T> vu[5], vb[4];
 #pragma unroll
             for (int c = 0; c < 5; c++) vu[c] = *reinterpret_cast<const V16<T> *>(u + cu<FR>(c) * N + t.kb + ro);
 #pragma unroll
             for (int c = 0; c < 3; c++) vb[c] = *reinterpret_cast<const V16<T> *>(b + cb<FR>(c) * N + t.kb + ro);
             vb[3] = *reinterpret_cast<const V16<T> *>(b + Comp<FR>::b[2] * N + t.kpb + ro);
-            const T b2n = b[Comp<FR>::b[1] * N + t.kb + (long)i * f.si + jn];
+            T b2n = (T)0;
+            if (last) b2n = b[Comp<FR>::b[1] * N + t.kb + (long)i * f.si + jn];
 #pragma unroll
-            for (int s = 0; s < W; s++) {
+            for (int v = 0; v < VE; v++) {
+                const int s = h * VE + v;
 #pragma unroll
-                for (int c = 0; c < 5; c++) su[(s * 5 + c) * n + i] = vget<T>(vu[c], s);
+                for (int c = 0; c < 5; c++) su[(s * 5 + c) * n + i] = vget<T>(vu[c], v);
 #pragma unroll
-                for (int c = 0; c < 3; c++) sb[(s * 3 + c) * n + i] = vget<T>(vb[c], s);
-                sbk[s * n + i] = vget<T>(vb[3], s);
+                for (int c = 0; c < 3; c++) sb[(s * 3 + c) * n + i] = vget<T>(vb[c], v);
+                sbk[s * n + i] = vget<T>(vb[3], v);
             }
-            sb2n[i] = b2n;
+            if (last) sb2n[i] = b2n;
         }
     } else {
         // general tile (partial, or vectors not aligned): one value per item
@@ -281,22 +326,31 @@ __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__re
         const int lc = l % L;  // lanes >= L shadow lane l % L and store nothing
         const int src_next = lc + 1 == L ? 0 : lc + 1, src_prev = lc == 0 ? L - 1 : lc - 1;
         const T dt = *dtp;
+        // the lane's K cells as 16-byte LDS reads (lane stride K values:
+        // scalar reads of that stride collide in the banks)
         T uu[K][5], bxc[K][3];
 #pragma unroll
-        for (int m = 0; m < 5; m++)
+        for (int m = 0; m < 5; m++) {
+            T x[K];
+            load_cells<T, K>(su + (w * 5 + m) * n + lc * K, x);
 #pragma unroll
-            for (int q = 0; q < K; q++) uu[q][m] = su[(w * 5 + m) * n + lc * K + q];
+            for (int q = 0; q < K; q++) uu[q][m] = x[q];
+        }
+        {
+            T b1[K], b1p[K], b2[K], b2p[K], b3[K], b3p[K];
+            load_cells<T, K>(sb + (w * 3 + 0) * n + lc * K, b1);
+            load_cells<T, K>(sb + (w * 3 + 1) * n + lc * K, b2);
+            load_cells<T, K>((w + 1 < ns ? sb + ((w + 1) * 3 + 1) * n : sb2n) + lc * K, b2p);
+            load_cells<T, K>(sb + (w * 3 + 2) * n + lc * K, b3);
+            load_cells<T, K>(sbk + w * n + lc * K, b3p);
+            shift_next<T, K>(b1, b1p, src_next);  // b1 at i+1, periodic along the pencil
 #pragma unroll
-        for (int q = 0; q < K; q++) {
-            const int i = lc * K + q, ip = i + 1 == n ? 0 : i + 1;
-            const T b1 = sb[(w * 3 + 0) * n + i], b1p = sb[(w * 3 + 0) * n + ip];
-            const T b2 = sb[(w * 3 + 1) * n + i];
-            const T b2p = w + 1 < ns ? sb[((w + 1) * 3 + 1) * n + i] : sb2n[i];
-            const T b3 = sb[(w * 3 + 2) * n + i], b3p = sbk[w * n + i];
-            // face B to cell centre (fluidx.c:71-81)
-            bxc[q][0] = (T)0.5 * b1 + (T)0.5 * b1p;
-            bxc[q][1] = (T)0.5 * b2 + b2p / (T)2;
-            bxc[q][2] = (T)0.5 * b3 + b3p / (T)2;
+            for (int q = 0; q < K; q++) {
+                // face B to cell centre (fluidx.c:71-81)
+                bxc[q][0] = (T)0.5 * b1[q] + (T)0.5 * b1p[q];
+                bxc[q][1] = (T)0.5 * b2[q] + b2p[q] / (T)2;
+                bxc[q][2] = (T)0.5 * b3[q] + b3p[q] / (T)2;
+            }
         }
         bool exact = true;
         if constexpr (sizeof(T) == 4 && FASTDIV) {
@@ -318,9 +372,12 @@ __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__re
         if (!exact) tvd1_pencil<T, K, false>(uu, bxc, dt, src_next, src_prev);
         if (l < L) {
 #pragma unroll
-            for (int m = 0; m < 5; m++)
+            for (int m = 0; m < 5; m++) {
+                T x[K];
 #pragma unroll
-                for (int q = 0; q < K; q++) su[(w * 5 + m) * n + lc * K + q] = uu[q][m];
+                for (int q = 0; q < K; q++) x[q] = uu[q][m];
+                store_cells<T, K>(su + (w * 5 + m) * n + lc * K, x);
+            }
         }
     }
     __syncthreads();
@@ -374,18 +431,21 @@ __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__re
         }
     } else if (FR != 0 && t.vec) {
         // component by component (the rows of one component leave together,
-        // which lets L2 merge the 16-byte row stores of neighbouring tiles):
-        // gather the W slots of row i
+        // which lets L2 merge the row stores of neighbouring tiles): a thread
+        // gathers the VE slots of one vector of row i, as in the loader
+        const int H = ns / VE;
 #pragma unroll
         for (int c = 0; c < 5; c++) {
             T *uc = u + cu<FR>(c) * N + t.j0 + t.kb;
-            for (int i = tid; i < n; i += nt) {
+            for (int e = tid; e < n * H; e += nt) {
+                const int i = H == 2 ? e >> 1 : e, h = e - i * H;
+                const T *sv = su + (h * VE * 5 + c) * n + i;
                 V16<T> v;
                 if constexpr (sizeof(T) == 4)
-                    v = make_float4(su[c * n + i], su[(5 + c) * n + i], su[(10 + c) * n + i], su[(15 + c) * n + i]);
+                    v = make_float4(sv[0], sv[5 * n], sv[10 * n], sv[15 * n]);
                 else
-                    v = make_double2(su[c * n + i], su[(5 + c) * n + i]);
-                *reinterpret_cast<V16<T> *>(uc + (long)i * f.si) = v;
+                    v = make_double2(sv[0], sv[5 * n]);
+                *reinterpret_cast<V16<T> *>(uc + (long)i * f.si + h * VE) = v;
             }
         }
     } else {
@@ -675,41 +735,6 @@ __device__ __forceinline__ void advect_soa_tile(const T *__restrict__ u, const T
 // ---------------------------------------------------------------------------
 template <int W> __host__ __device__ constexpr int advect_x_words(int n) { return (3 * W + 1) * n; }
 
-template <typename T, int K>
-__device__ __forceinline__ void load_cells(const T *__restrict__ p, T (&o)[K])
-{
-    constexpr int VE = Vec16<T>::n;
-    if constexpr (K % VE == 0) {
-#pragma unroll
-        for (int v = 0; v < K / VE; v++) {
-            const V16<T> x = *reinterpret_cast<const V16<T> *>(p + v * VE);
-#pragma unroll
-            for (int e = 0; e < VE; e++) o[v * VE + e] = vget<T>(x, e);
-        }
-    } else {
-#pragma unroll
-        for (int q = 0; q < K; q++) o[q] = p[q];
-    }
-}
-
-template <typename T, int K>
-__device__ __forceinline__ void store_cells(T *__restrict__ p, const T (&o)[K])
-{
-    constexpr int VE = Vec16<T>::n;
-    if constexpr (K % VE == 0 && sizeof(T) == 4) {
-#pragma unroll
-        for (int v = 0; v < K / VE; v++)
-            *reinterpret_cast<float4 *>(p + v * VE) = make_float4(o[v * VE], o[v * VE + 1], o[v * VE + 2], o[v * VE + 3]);
-    } else if constexpr (K % VE == 0) {
-#pragma unroll
-        for (int v = 0; v < K / VE; v++)
-            *reinterpret_cast<double2 *>(p + v * VE) = make_double2(o[v * VE], o[v * VE + 1]);
-    } else {
-#pragma unroll
-        for (int q = 0; q < K; q++) p[q] = o[q];
-    }
-}
-
 // vec: 16-byte loads apply (rows and planes aligned, K a multiple of the
 // vector width); otherwise the cells are read one by one
 template <typename T, int K, int W, bool VEC>
@@ -780,8 +805,7 @@ __device__ __forceinline__ void advect_x_tile(const T *__restrict__ u, const T *
         for (int q = 0; q < K; q++) vs[q] = (T)(((double)(vxm[q] + vxn[q]) + 2.0 * (double)vx[q]) / 4.0);
         tvdb_pencil<T, K>(bi, vs, dt, bnew, F, src_next, src_prev);
         if (l < L) {
-#pragma unroll
-            for (int q = 0; q < K; q++) X[it * n + cell0 + q] = F[q];
+            store_cells<T, K>(X + it * n + cell0, F);  // 16-byte LDS writes when K allows
             // new b2 (FA) / b3 (FB at k) of the tile's own pencils
             if (it < ns) {
                 if constexpr (VEC) store_cells<T, K>(b2o + pf + cell0, bnew);
@@ -806,14 +830,18 @@ __device__ __forceinline__ void advect_x_tile(const T *__restrict__ u, const T *
         else
 #pragma unroll
             for (int q = 0; q < K; q++) b1[q] = b1i[p + q];
+        T fa0[K], fa1[K], fb0[K], fb1[K];
+        load_cells<T, K>(X + s * n + cell0, fa0);
+        load_cells<T, K>(X + (s + 1) * n + cell0, fa1);
+        load_cells<T, K>(X + (ns + 1 + s) * n + cell0, fb0);
+        load_cells<T, K>(X + (2 * ns + 1 + s) * n + cell0, fb1);
 #pragma unroll
         for (int q = 0; q < K; q++) {
-            const long i = cell0 + q;
             T v = b1[q];
-            v = v - X[s * n + i];
-            v = v + X[(s + 1) * n + i];
-            v = v - X[(ns + 1 + s) * n + i];
-            v = v + X[(2 * ns + 1 + s) * n + i];
+            v = v - fa0[q];
+            v = v + fa1[q];
+            v = v - fb0[q];
+            v = v + fb1[q];
             b1[q] = v;
         }
         if constexpr (VEC) store_cells<T, K>(b1o + p, b1);
@@ -896,10 +924,7 @@ __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__rest
 #pragma unroll
                     for (int q = 0; q < K; q++) u[m * N + p + q] = o[q];
                 }
-                if constexpr (CFLM != 0) {
-#pragma unroll
-                    for (int q = 0; q < K; q++) su[(w * 5 + m) * n + cell0 + q] = o[q];
-                }
+                if constexpr (CFLM != 0) store_cells<T, K>(su + (w * 5 + m) * n + cell0, o);
             }
         }
     }
@@ -920,9 +945,11 @@ __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__rest
                 load(b + 2 * N + pk, b3p);
                 shift_next<T, K>(b1, b1p, src_next);
             }
+            T um[5][K];
+#pragma unroll
+            for (int m = 0; m < 5; m++) load_cells<T, K>(su + (w * 5 + m) * n + cell0, um[m]);
 #pragma unroll
             for (int q = 0; q < K; q++) {
-                const T *ui = su + w * 5 * n + cell0 + q;
                 T B;
                 if constexpr (CFLM == 2) {
                     const T bxa = (T)((double)(b1[q] + b1p[q]) / 2.0);
@@ -932,7 +959,7 @@ __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__rest
                 } else {
                     B = b1[q] * b1[q] + b2[q] * b2[q] + b3[q] * b3[q];
                 }
-                const T sp = cfl_speed<T>(ui[0], ui[n], ui[2 * n], ui[3 * n], ui[4 * n], B);
+                const T sp = cfl_speed<T>(um[0][q], um[1][q], um[2][q], um[3][q], um[4][q], B);
                 c = c > sp ? c : sp;  // max_2num (calcfl.c:55)
             }
         }
