@@ -321,9 +321,13 @@ static hipError_t fluidx_soa_k(T *u, const T *b, const SoaBox &bx, const T *dt, 
     if constexpr (FR != 0) {
         // y/z frames: tiles two vectors wide (32-byte rows: half the L2
         // requests of 16-byte rows) where two such tiles fit a CU's LDS
+        // (measured on MI355X at 256^3: 0.356-0.363 ms against 0.373 with one
+        // vector; four vectors, 148 KB of LDS and one tile per CU, 0.50 ms).
+        // MHD_YZ_W=1 selects the one-vector tiles.
+        static const int yzw = [] { const char *v = getenv("MHD_YZ_W"); return v ? atoi(v) : 2; }();
         constexpr int W2 = 2 * W;
         const size_t lds2 = sizeof(T) * fluidx_soa_words<W2>(bx.f.ni);
-        if (!fast_div_enabled() && lds2 <= 80 * 1024) {
+        if (yzw >= 2 && !fast_div_enabled() && lds2 <= 80 * 1024) {
             const int ntj2 = (bx.f.nj + W2 - 1) / W2;
             hipLaunchKernelGGL((fluidx_soa_kernel<T, K, W2, FR, false>), dim3(ntj2 * bx.f.nk), dim3(64 * W2), lds2, st,
                                u, b, bx, dt, ntj2, (T *)nullptr);
