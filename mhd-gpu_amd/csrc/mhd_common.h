@@ -204,6 +204,17 @@ __device__ __forceinline__ float div_by(float a, const Recip &r, bool &ok)
     return fmaf(e, r.y, q);
 }
 
+// Quotient a/d from a refined reciprocal, no range checks: div_by's
+// arithmetic, for callers that checked the operands' ranges in bulk.
+__device__ __forceinline__ float quot_by(float a, const Recip &r)
+{
+    float q = a * r.y;
+    float e = fmaf(-r.d, q, a);
+    q = fmaf(e, r.y, q);
+    e = fmaf(-r.d, q, a);
+    return fmaf(e, r.y, q);
+}
+
 // mhdflux_cell with the four divisions by u1 sharing one reciprocal (FP32)
 __device__ __forceinline__ void mhdflux_cell_fast(const float *u, const float *b, float *v, float &ci, bool &ok)
 {

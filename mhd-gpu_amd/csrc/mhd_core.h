@@ -31,6 +31,41 @@ __device__ __forceinline__ bool mhdflux_pencil(const T (&uu)[K][5], const T (&bx
             for (int q = 0; q < K; q++)
 #pragma unroll
                 for (int m = 0; m < 5; m++) v[q][m] = div_by(v[q][m], rc, ok);
+        } else if constexpr (sizeof(T) == 4) {
+            // v/c for the 5K fluxes of the lane: c is uniform over the
+            // wave, so its reciprocal is formed once and each quotient takes
+            // the remaining five operations of the IEEE division sequence;
+            // that is exact while no operand scaling would trigger (div_by).
+            // The operand ranges are checked in bulk: c finite with exponent
+            // in [-30, 30]; the largest |v| below 2^60 (which also rules out
+            // infinities) and the smallest exponent at least -60 (zero has
+            // exponent 0 and divides exactly, up to the sign of the zero; NaN
+            // gives NaN either way).  The wave falls back to true divisions
+            // if any lane is outside.
+            const int ec = __builtin_amdgcn_frexp_expf(c);
+            const bool cok = ec >= -30 && ec <= 30 && c < __builtin_inff();
+            float vmax = 0.0f;
+            int emin = 0;
+#pragma unroll
+            for (int q = 0; q < K; q++)
+#pragma unroll
+                for (int m = 0; m < 5; m++) {
+                    vmax = fmaxf(vmax, fabsf(v[q][m]));
+                    emin = min(emin, __builtin_amdgcn_frexp_expf(v[q][m]));
+                }
+            if (__all(cok && vmax < 0x1p60f && emin >= -60)) {
+                const float y0 = __builtin_amdgcn_rcpf(c);
+                const Recip rc{c, fmaf(fmaf(-c, y0, 1.0f), y0, y0)};
+#pragma unroll
+                for (int q = 0; q < K; q++)
+#pragma unroll
+                    for (int m = 0; m < 5; m++) v[q][m] = quot_by(v[q][m], rc);
+            } else {
+#pragma unroll
+                for (int q = 0; q < K; q++)
+#pragma unroll
+                    for (int m = 0; m < 5; m++) v[q][m] = v[q][m] / c;
+            }
         } else {
 #pragma unroll
             for (int q = 0; q < K; q++)
