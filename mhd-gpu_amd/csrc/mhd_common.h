@@ -236,6 +236,38 @@ __device__ __forceinline__ void mhdflux_cell_fast(const float *u, const float *b
     ci = (float)(fabs((double)vx) + sqrt_ranged(fabs((double)div_by(B + g * p, r1, ok))));
 }
 
+// mhdflux_cell (FP32) with the four divisions by u1 through one refined
+// reciprocal (quot_by); the operand ranges are not checked here but folded
+// into the caller's bulk check: ok takes |u1| in [2^-31, 2^30), vmax the
+// largest |numerator| and emin the smallest numerator exponent.
+__device__ __forceinline__ void mhdflux_cell_bulk(const float *u, const float *b, float *v, float &ci, bool &ok,
+                                                  float &vmax, int &emin)
+{
+    const float u1 = u[0], u2 = u[1], u3 = u[2], u4 = u[3], u5 = u[4];
+    const float b1 = b[0], b2 = b[1], b3 = b[2];
+    ok = ok && fabsf(u1) < 0x1p30f && __builtin_amdgcn_frexp_expf(u1) >= -30;
+    const float y0 = __builtin_amdgcn_rcpf(u1);
+    const Recip r1{u1, fmaf(fmaf(-u1, y0, 1.0f), y0, y0)};
+    const float vx = quot_by(u2, r1);
+    const float S = u2 * u2 + u3 * u3 + u4 * u4;
+    const float B = b1 * b1 + b2 * b2 + b3 * b3;
+    const float g = gamma_c<float>();
+    const float ps = (float)(((double)u5 - (double)quot_by(S, r1) / 2.0) * ((double)g - 1.0) +
+                             (2.0 - (double)g) * (double)B / 2.0);
+    v[0] = u2;
+    v[1] = u2 * vx + ps - b1 * b1;
+    v[2] = u3 * vx - b2 * b1;
+    v[3] = u4 * vx - b3 * b1;
+    const float bm = b1 * (b1 * u2 + b2 * u3 + b3 * u4);
+    v[4] = (u5 + ps) * vx - quot_by(bm, r1);
+    const float p = (float)((double)ps - (double)B / 2.0);
+    const float w = B + g * p;
+    ci = (float)(fabs((double)vx) + sqrt_ranged(fabs((double)quot_by(w, r1))));
+    vmax = fmaxf(fmaxf(vmax, fabsf(u2)), fmaxf(fmaxf(fabsf(S), fabsf(bm)), fabsf(w)));
+    emin = min(min(emin, min(__builtin_amdgcn_frexp_expf(u2), __builtin_amdgcn_frexp_expf(S))),
+               min(__builtin_amdgcn_frexp_expf(bm), __builtin_amdgcn_frexp_expf(w)));
+}
+
 // Per-cell fast speed of calcfl.c:45-54 (face B at the cell, or the cell
 // average of OpenMP/mhd.f90:112-120 when the caller passes averaged b).
 template <typename T>

@@ -14,14 +14,39 @@ __device__ __forceinline__ bool mhdflux_pencil(const T (&uu)[K][5], const T (&bx
 {
     bool ok = true;
     T cl = (T)0;
+    if constexpr (!FAST && sizeof(T) == 4) {
+        // the divisions by u1 through one reciprocal per cell (exact while no
+        // operand scaling would trigger, as for the normalisation below);
+        // the wave recomputes its cells with true divisions if any operand
+        // of any lane is out of range
+        bool uok = true;
+        float vmax = 0.0f;
+        int emin = 0;
 #pragma unroll
-    for (int q = 0; q < K; q++) {
-        T ci;
-        if constexpr (FAST)
-            mhdflux_cell_fast(uu[q], bx[q], v[q], ci, ok);
-        else
-            mhdflux_cell<T>(uu[q], bx[q], v[q], ci);
-        cl = fmax(cl, ci);
+        for (int q = 0; q < K; q++) {
+            T ci;
+            mhdflux_cell_bulk(uu[q], bx[q], v[q], ci, uok, vmax, emin);
+            cl = fmax(cl, ci);
+        }
+        if (!__all(uok && vmax < 0x1p60f && emin >= -60)) {
+            cl = (T)0;
+#pragma unroll
+            for (int q = 0; q < K; q++) {
+                T ci;
+                mhdflux_cell<T>(uu[q], bx[q], v[q], ci);
+                cl = fmax(cl, ci);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            T ci;
+            if constexpr (FAST)
+                mhdflux_cell_fast(uu[q], bx[q], v[q], ci, ok);
+            else
+                mhdflux_cell<T>(uu[q], bx[q], v[q], ci);
+            cl = fmax(cl, ci);
+        }
     }
     c = wave_max(cl);
     if (c > (T)0) {

@@ -52,8 +52,9 @@ __global__ void __launch_bounds__(64 * W)
                                  reinterpret_cast<T *>(smem_raw));
 }
 
+// at most 128 VGPRs for K <= 4 (four waves per SIMD)
 template <typename T, int K, int W, bool VEC, int CFLM>
-__global__ void __launch_bounds__(64 * W)
+__global__ void __launch_bounds__(64 * W, K <= 4 ? 4 : 1)
     fluidx_x_kernel(T *__restrict__ u, const T *__restrict__ b, SoaBox bx, const T *__restrict__ dtp, int ntj,
                     T *__restrict__ cfl_part)
 {
