@@ -181,31 +181,18 @@ __device__ __forceinline__ void mhdflux_cell(const T *u, const T *b, T *v, T &ci
 // compiler's IEEE division runs (v_div_scale ... v_div_fmas) when its operand
 // scaling does not trigger, i.e. for normal d with |exponent| <= 30 and a of
 // |exponent| <= 60 or a == 0 (the quotient then equals a/d; a zero quotient
-// may differ in sign only).  `ok` collects the range checks; a caller whose
-// ok ends false recomputes with true divisions.
+// may differ in sign only).  Callers check those ranges in bulk and fall
+// back to true divisions outside them.
 struct Recip {
     float d, y;
 };
-__device__ __forceinline__ Recip recip_of(float d, bool &ok)
+__device__ __forceinline__ Recip recip_of(float d)
 {
-    const int e = __builtin_amdgcn_frexp_expf(d);
-    ok = ok && d != 0.0f && (unsigned)(e + 30) <= 60u;
     const float y0 = __builtin_amdgcn_rcpf(d);
-    const float r = fmaf(-d, y0, 1.0f);
-    return {d, fmaf(r, y0, y0)};
-}
-__device__ __forceinline__ float div_by(float a, const Recip &r, bool &ok)
-{
-    ok = ok && (unsigned)(__builtin_amdgcn_frexp_expf(a) + 60) <= 120u;
-    float q = a * r.y;
-    float e = fmaf(-r.d, q, a);
-    q = fmaf(e, r.y, q);
-    e = fmaf(-r.d, q, a);
-    return fmaf(e, r.y, q);
+    return {d, fmaf(fmaf(-d, y0, 1.0f), y0, y0)};
 }
 
-// Quotient a/d from a refined reciprocal, no range checks: div_by's
-// arithmetic, for callers that checked the operands' ranges in bulk.
+// a/d from the refined reciprocal of d (ranges checked by the caller)
 __device__ __forceinline__ float quot_by(float a, const Recip &r)
 {
     float q = a * r.y;
@@ -213,27 +200,6 @@ __device__ __forceinline__ float quot_by(float a, const Recip &r)
     q = fmaf(e, r.y, q);
     e = fmaf(-r.d, q, a);
     return fmaf(e, r.y, q);
-}
-
-// mhdflux_cell with the four divisions by u1 sharing one reciprocal (FP32)
-__device__ __forceinline__ void mhdflux_cell_fast(const float *u, const float *b, float *v, float &ci, bool &ok)
-{
-    const float u1 = u[0], u2 = u[1], u3 = u[2], u4 = u[3], u5 = u[4];
-    const float b1 = b[0], b2 = b[1], b3 = b[2];
-    const Recip r1 = recip_of(u1, ok);
-    const float vx = div_by(u2, r1, ok);
-    const float S = u2 * u2 + u3 * u3 + u4 * u4;
-    const float B = b1 * b1 + b2 * b2 + b3 * b3;
-    const float g = gamma_c<float>();
-    const float q = div_by(S, r1, ok);
-    const float ps = (float)(((double)u5 - (double)q / 2.0) * ((double)g - 1.0) + (2.0 - (double)g) * (double)B / 2.0);
-    v[0] = u2;
-    v[1] = u2 * vx + ps - b1 * b1;
-    v[2] = u3 * vx - b2 * b1;
-    v[3] = u4 * vx - b3 * b1;
-    v[4] = (u5 + ps) * vx - div_by(b1 * (b1 * u2 + b2 * u3 + b3 * u4), r1, ok);
-    const float p = (float)((double)ps - (double)B / 2.0);
-    ci = (float)(fabs((double)vx) + sqrt_ranged(fabs((double)div_by(B + g * p, r1, ok))));
 }
 
 // mhdflux_cell (FP32) with the four divisions by u1 through one refined
@@ -246,8 +212,7 @@ __device__ __forceinline__ void mhdflux_cell_bulk(const float *u, const float *b
     const float u1 = u[0], u2 = u[1], u3 = u[2], u4 = u[3], u5 = u[4];
     const float b1 = b[0], b2 = b[1], b3 = b[2];
     ok = ok && fabsf(u1) < 0x1p30f && __builtin_amdgcn_frexp_expf(u1) >= -30;
-    const float y0 = __builtin_amdgcn_rcpf(u1);
-    const Recip r1{u1, fmaf(fmaf(-u1, y0, 1.0f), y0, y0)};
+    const Recip r1 = recip_of(u1);
     const float vx = quot_by(u2, r1);
     const float S = u2 * u2 + u3 * u3 + u4 * u4;
     const float B = b1 * b1 + b2 * b2 + b3 * b3;

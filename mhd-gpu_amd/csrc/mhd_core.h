@@ -7,14 +7,13 @@
 
 namespace mhd {
 
-// FAST (FP32 only): the divisions of mhdflux run through div_by; the
-// returned flag says whether every operand was in its exact range.
-template <typename T, int K, bool FAST>
-__device__ __forceinline__ bool mhdflux_pencil(const T (&uu)[K][5], const T (&bx)[K][3], T (&v)[K][5], T &c)
+// Unnormalised fluxes v and the pencil's freezing speed c (mhdflux.c:21-69),
+// then v/c.
+template <typename T, int K>
+__device__ __forceinline__ void mhdflux_pencil(const T (&uu)[K][5], const T (&bx)[K][3], T (&v)[K][5], T &c)
 {
-    bool ok = true;
     T cl = (T)0;
-    if constexpr (!FAST && sizeof(T) == 4) {
+    if constexpr (sizeof(T) == 4) {
         // the divisions by u1 through one reciprocal per cell (exact while no
         // operand scaling would trigger, as for the normalisation below);
         // the wave recomputes its cells with true divisions if any operand
@@ -41,26 +40,17 @@ __device__ __forceinline__ bool mhdflux_pencil(const T (&uu)[K][5], const T (&bx
 #pragma unroll
         for (int q = 0; q < K; q++) {
             T ci;
-            if constexpr (FAST)
-                mhdflux_cell_fast(uu[q], bx[q], v[q], ci, ok);
-            else
-                mhdflux_cell<T>(uu[q], bx[q], v[q], ci);
+            mhdflux_cell<T>(uu[q], bx[q], v[q], ci);
             cl = fmax(cl, ci);
         }
     }
     c = wave_max(cl);
     if (c > (T)0) {
-        if constexpr (FAST) {
-            const Recip rc = recip_of(c, ok);
-#pragma unroll
-            for (int q = 0; q < K; q++)
-#pragma unroll
-                for (int m = 0; m < 5; m++) v[q][m] = div_by(v[q][m], rc, ok);
-        } else if constexpr (sizeof(T) == 4) {
+        if constexpr (sizeof(T) == 4) {
             // v/c for the 5K fluxes of the lane: c is uniform over the
             // wave, so its reciprocal is formed once and each quotient takes
             // the remaining five operations of the IEEE division sequence;
-            // that is exact while no operand scaling would trigger (div_by).
+            // that is exact while no operand scaling would trigger (Recip).
             // The operand ranges are checked in bulk: c finite with exponent
             // in [-30, 30]; the largest |v| below 2^60 (which also rules out
             // infinities) and the smallest exponent at least -60 (zero has
@@ -79,8 +69,7 @@ __device__ __forceinline__ bool mhdflux_pencil(const T (&uu)[K][5], const T (&bx
                     emin = min(emin, __builtin_amdgcn_frexp_expf(v[q][m]));
                 }
             if (__all(cok && vmax < 0x1p60f && emin >= -60)) {
-                const float y0 = __builtin_amdgcn_rcpf(c);
-                const Recip rc{c, fmaf(fmaf(-c, y0, 1.0f), y0, y0)};
+                const Recip rc = recip_of(c);
 #pragma unroll
                 for (int q = 0; q < K; q++)
 #pragma unroll
@@ -98,20 +87,18 @@ __device__ __forceinline__ bool mhdflux_pencil(const T (&uu)[K][5], const T (&bx
                 for (int m = 0; m < 5; m++) v[q][m] = v[q][m] / c;
         }
     }
-    return ok;
 }
 
 // tvd1 on a pencil held K cells per lane; uu is updated in place.
-template <typename T, int K, bool FAST = false>
-__device__ __forceinline__ bool tvd1_pencil(T (&uu)[K][5], const T (&bx)[K][3], T dt, int src_next, int src_prev)
+template <typename T, int K>
+__device__ __forceinline__ void tvd1_pencil(T (&uu)[K][5], const T (&bx)[K][3], T dt, int src_next, int src_prev)
 {
-    bool ok = true;
     const T h = (T)(-1.0 * (double)dt / 2.0);
     const T mdt = (T)(-1.0 * (double)dt);
     T v[K][5], c;
     T u1[K][5];
     // first half step (tvd1.c:34-49)
-    ok = mhdflux_pencil<T, K, FAST>(uu, bx, v, c) && ok;
+    mhdflux_pencil<T, K>(uu, bx, v, c);
 #pragma unroll
     for (int m = 0; m < 5; m++) {
         T wl[K], fr[K], wln[K], fl[K], fx[K], fxp[K];
@@ -132,7 +119,7 @@ __device__ __forceinline__ bool tvd1_pencil(T (&uu)[K][5], const T (&bx)[K][3], 
         for (int q = 0; q < K; q++) u1[q][m] = uu[q][m] + h * (fx[q] - fxp[q]);
     }
     // full step with limited fluxes (tvd1.c:52-116)
-    ok = mhdflux_pencil<T, K, FAST>(u1, bx, v, c) && ok;
+    mhdflux_pencil<T, K>(u1, bx, v, c);
 #pragma unroll
     for (int m = 0; m < 5; m++) {
         // tvd1.c:59-90 forms dfrp = 0.5*fr[i+1] + (-0.5)*fr[i] and
@@ -175,7 +162,6 @@ __device__ __forceinline__ bool tvd1_pencil(T (&uu)[K][5], const T (&bx)[K][3], 
 #pragma unroll
         for (int q = 0; q < K; q++) uu[q][m] = uu[q][m] + mdt * (fx[q] - fxp[q]);
     }
-    return ok;
 }
 
 // tvdb on a pencil held K cells per lane (tvdb.c:10-91): bin -> bnew, and

@@ -18,28 +18,14 @@
 
 namespace mhd {
 
-template <typename T, int K, int W, int FR, bool FASTDIV, int CFLM = 0>
+// y/z frames (LDS tiles)
+template <typename T, int K, int W, int FR>
 __global__ void __launch_bounds__(64 * W)
-    fluidx_soa_kernel(T *__restrict__ u, const T *__restrict__ b, SoaBox bx, const T *__restrict__ dtp, int ntj,
-                      T *__restrict__ cfl_part)
+    fluidx_soa_kernel(T *__restrict__ u, const T *__restrict__ b, SoaBox bx, const T *__restrict__ dtp, int ntj)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    fluidx_soa_tile<T, K, W, FR, FASTDIV, CFLM>(u, b, bx, dtp, xcd_tile(blockIdx.x, gridDim.x), ntj,
-                                                reinterpret_cast<T *>(smem_raw), cfl_part);
-}
-
-// MHD_FAST_DIV=1 selects the fluidx build whose mhdflux divisions share
-// reciprocals (bit-exact, see div_by).  It is off by default: on MI355X at
-// 256^3 it measured 0.558 ms per fluidx against 0.406 ms with true
-// divisions, the fewer VALU operations losing to 176 instead of 124 VGPRs
-// (2 instead of 4 waves per SIMD).
-static bool fast_div_enabled()
-{
-    static const bool on = [] {
-        const char *v = getenv("MHD_FAST_DIV");
-        return v && v[0] == '1';
-    }();
-    return on;
+    fluidx_soa_tile<T, K, W, FR>(u, b, bx, dtp, xcd_tile(blockIdx.x, gridDim.x), ntj,
+                                 reinterpret_cast<T *>(smem_raw));
 }
 
 template <typename T, int K, int W, int FR>
@@ -256,22 +242,12 @@ template <typename T, int K, int FR> static hipError_t soa_attr()
 {
     constexpr int W = soa_w<T>();
     static const hipError_t e = [] {
-        hipError_t a = hipFuncSetAttribute((const void *)fluidx_soa_kernel<T, K, W, FR, true>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        if (a != hipSuccess) return a;
-        a = hipFuncSetAttribute((const void *)fluidx_soa_kernel<T, K, W, FR, false>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        if (a != hipSuccess) return a;
-        if constexpr (FR == 0) {
-            for (const void *k : {(const void *)fluidx_soa_kernel<T, K, W, 0, false, 1>,
-                                  (const void *)fluidx_soa_kernel<T, K, W, 0, false, 2>}) {
-                a = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
+        if constexpr (FR != 0) {
+            for (const void *k : {(const void *)fluidx_soa_kernel<T, K, W, FR>,
+                                  (const void *)fluidx_soa_kernel<T, K, 2 * W, FR>}) {
+                const hipError_t a = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
                 if (a != hipSuccess) return a;
             }
-        } else {
-            a = hipFuncSetAttribute((const void *)fluidx_soa_kernel<T, K, 2 * W, FR, false>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-            if (a != hipSuccess) return a;
         }
         return hipFuncSetAttribute((const void *)advect_soa_kernel<T, K, W, FR>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
@@ -290,36 +266,22 @@ static hipError_t fluidx_soa_k(T *u, const T *b, const SoaBox &bx, const T *dt, 
     const size_t lds = sizeof(T) * fluidx_soa_words<W>(bx.f.ni);
     const dim3 grid(ntj * bx.f.nk), block(64 * W);
     if constexpr (FR == 0) {
-        if (!fast_div_enabled()) {
-            // x frame: direct loads, no LDS staging
-            constexpr bool kvec = K % (16 / sizeof(T)) == 0;
-            const size_t xl = sizeof(T) * fluidx_x_words<W>(bx.f.ni, cflm);
+        // x frame: direct loads, no LDS staging (LDS only for the fused CFL)
+        constexpr bool kvec = K % (16 / sizeof(T)) == 0;
+        const size_t xl = sizeof(T) * fluidx_x_words<W>(bx.f.ni, cflm);
 #define MHD_FX(V, C)                                                                                                 \
     hipLaunchKernelGGL((fluidx_x_kernel<T, K, W, V, C>), grid, block, xl, st, u, b, bx, dt, ntj, cfl_part)
-            if (kvec && bx.vec_ok) {
-                if (cflm == 1) MHD_FX(kvec, 1);
-                else if (cflm == 2) MHD_FX(kvec, 2);
-                else MHD_FX(kvec, 0);
-            } else {
-                if (cflm == 1) MHD_FX(false, 1);
-                else if (cflm == 2) MHD_FX(false, 2);
-                else MHD_FX(false, 0);
-            }
+        if (kvec && bx.vec_ok) {
+            if (cflm == 1) MHD_FX(kvec, 1);
+            else if (cflm == 2) MHD_FX(kvec, 2);
+            else MHD_FX(kvec, 0);
+        } else {
+            if (cflm == 1) MHD_FX(false, 1);
+            else if (cflm == 2) MHD_FX(false, 2);
+            else MHD_FX(false, 0);
+        }
 #undef MHD_FX
-            return hipGetLastError();
-        }
-        if (cflm == 1) {
-            hipLaunchKernelGGL((fluidx_soa_kernel<T, K, W, 0, false, 1>), grid, block, lds, st, u, b, bx, dt, ntj,
-                               cfl_part);
-            return hipGetLastError();
-        }
-        if (cflm == 2) {
-            hipLaunchKernelGGL((fluidx_soa_kernel<T, K, W, 0, false, 2>), grid, block, lds, st, u, b, bx, dt, ntj,
-                               cfl_part);
-            return hipGetLastError();
-        }
-    }
-    if constexpr (FR != 0) {
+    } else {
         // y/z frames: tiles two vectors wide (32-byte rows: half the L2
         // requests of 16-byte rows) where two such tiles fit a CU's LDS
         // (measured on MI355X at 256^3: 0.356-0.363 ms against 0.373 with one
@@ -328,19 +290,14 @@ static hipError_t fluidx_soa_k(T *u, const T *b, const SoaBox &bx, const T *dt, 
         static const int yzw = [] { const char *v = getenv("MHD_YZ_W"); return v ? atoi(v) : 2; }();
         constexpr int W2 = 2 * W;
         const size_t lds2 = sizeof(T) * fluidx_soa_words<W2>(bx.f.ni);
-        if (yzw >= 2 && !fast_div_enabled() && lds2 <= 80 * 1024) {
+        if (yzw >= 2 && lds2 <= 80 * 1024) {
             const int ntj2 = (bx.f.nj + W2 - 1) / W2;
-            hipLaunchKernelGGL((fluidx_soa_kernel<T, K, W2, FR, false>), dim3(ntj2 * bx.f.nk), dim3(64 * W2), lds2, st,
-                               u, b, bx, dt, ntj2, (T *)nullptr);
-            return hipGetLastError();
+            hipLaunchKernelGGL((fluidx_soa_kernel<T, K, W2, FR>), dim3(ntj2 * bx.f.nk), dim3(64 * W2), lds2, st, u, b,
+                               bx, dt, ntj2);
+        } else {
+            hipLaunchKernelGGL((fluidx_soa_kernel<T, K, W, FR>), grid, block, lds, st, u, b, bx, dt, ntj);
         }
     }
-    if (fast_div_enabled())
-        hipLaunchKernelGGL((fluidx_soa_kernel<T, K, W, FR, true>), grid, block, lds, st, u, b, bx, dt, ntj,
-                           (T *)nullptr);
-    else
-        hipLaunchKernelGGL((fluidx_soa_kernel<T, K, W, FR, false>), grid, block, lds, st, u, b, bx, dt, ntj,
-                           (T *)nullptr);
     return hipGetLastError();
 }
 

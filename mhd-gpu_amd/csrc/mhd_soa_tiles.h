@@ -205,7 +205,7 @@ __device__ __forceinline__ void store_cells(T *__restrict__ p, const T (&o)[K])
 // new state (calcfl.c:45-56, face B / cell-averaged B of OpenMP/mhd.f90:
 // 112-120), the block's max written to cfl_part[tile] - the sweep is the
 // step's last, so that is the next step's CFL
-template <typename T, int K, int W, int FR, bool FASTDIV = false, int CFLM = 0>
+template <typename T, int K, int W, int FR, int CFLM = 0>
 __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__restrict__ b, const SoaBox &bx,
                                                 const T *__restrict__ dtp, int tile, int ntj, T *smem,
                                                 T *__restrict__ cfl_part = nullptr)
@@ -352,24 +352,7 @@ __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__re
                 bxc[q][2] = (T)0.5 * b3[q] + b3p[q] / (T)2;
             }
         }
-        bool exact = true;
-        if constexpr (sizeof(T) == 4 && FASTDIV) {
-            // fast divisions; if any lane met an operand outside their exact
-            // range, redo the pencil with true divisions (wave-uniform branch)
-            exact = __all(tvd1_pencil<T, K, true>(uu, bxc, dt, src_next, src_prev));
-            if (!exact) {
-                // reload from LDS (the compiler must not keep the inputs live
-                // across the fast attempt for this rare case)
-                asm volatile("" ::: "memory");
-#pragma unroll
-                for (int m = 0; m < 5; m++)
-#pragma unroll
-                    for (int q = 0; q < K; q++) uu[q][m] = su[(w * 5 + m) * n + lc * K + q];
-            }
-        } else {
-            exact = false;
-        }
-        if (!exact) tvd1_pencil<T, K, false>(uu, bxc, dt, src_next, src_prev);
+        tvd1_pencil<T, K>(uu, bxc, dt, src_next, src_prev);
         if (l < L) {
 #pragma unroll
             for (int m = 0; m < 5; m++) {
@@ -911,7 +894,7 @@ __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__rest
                 bxc[q][2] = (T)0.5 * b3[q] + b3p[q] / (T)2;
             }
         }
-        tvd1_pencil<T, K, false>(uu, bxc, dt, src_next, src_prev);
+        tvd1_pencil<T, K>(uu, bxc, dt, src_next, src_prev);
         if (l < L) {
 #pragma unroll
             for (int m = 0; m < 5; m++) {
