@@ -46,6 +46,7 @@ struct BoxGroup {
     int act_off, act_n;
     int lds_f[3], lds_a[3];  // dynamic LDS bytes of fluidx / advect per frame
     int sa_off[3], sa_n[3], sb_off[3], sb_n[3];  // block items of the streamed y/z advectbyzx
+    int sab_nw[3];  // waves per block of the fused stream (its items in the sa list); 0: the two passes
     int tile_adv_n[3];  // leading items of item_off[fr] whose boxes run the tile advectbyzx
     long cells;
 };

@@ -195,6 +195,24 @@ __global__ void __launch_bounds__(256) batch_stream_b_kernel(const BoxDesc *__re
                                    wave);
 }
 
+// Fused y/z stream (schedule 3): one block per item (box, plane pair and
+// segment), sized for the group's widest box; the waves beyond a narrower
+// box's pencil tiles end at once (a barrier waits only for running waves)
+template <int FR, int NW>
+__global__ void __launch_bounds__(64 * NW) batch_stream_ab_kernel(const BoxDesc *__restrict__ boxes,
+                                                                  const WorkItem *__restrict__ items,
+                                                                  const float *__restrict__ dt, int flip)
+{
+    __shared__ float xch[64];
+    const WorkItem wi = items[xcd_tile(blockIdx.x, gridDim.x)];
+    const BoxDesc &bd = boxes[wi.box];
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (w >= bd.sb[FR].ntj) return;
+    const int nk2 = (bd.sb[FR].nk + 1) / 2;
+    advect_stream_ab<float, FR>(bd.u, flip ? bd.b2 : bd.b, flip ? bd.b : bd.b2, bd.sb[FR], dt + wi.box,
+                                wi.tile % nk2, wi.tile / nk2, w, xch);
+}
+
 // CFL over the active boxes: one block per (box, unit range); per-box max
 template <bool AVG>
 __global__ void __launch_bounds__(256) batch_cfl_kernel(const BoxDesc *__restrict__ boxes,
@@ -875,7 +893,9 @@ int Engine::build()
                                                       hipMemcpyHostToDevice, st_);
         if (e != hipSuccess) return hip_fail(e, "engine build items");
     }
-    // streamed y/z advectbyzx (schedule 3): per group and frame, one item per
+    // streamed y/z advectbyzx (schedule 3), per group and frame: the fused
+    // stream where every streamed box of the group allows it, one item per
+    // (box, plane pair and segment) block; else the two passes, one item per
     // block of four waves of each box, pass A and pass B in their own lists
     std::vector<WorkItem> sitems[2];
     for (size_t g = 0; g < groups_.size(); g++) {
@@ -883,11 +903,25 @@ int Engine::build()
         for (int fr = 1; fr < 3; fr++) {
             G.sa_off[fr] = (int)sitems[0].size();
             G.sb_off[fr] = (int)sitems[1].size();
+            int nw = 0;
+            bool fuse = true;
+            for (int oi = G.act_off; oi < G.act_off + G.act_n; oi++) {
+                if (!stream_now_[order[oi]]) continue;
+                const StreamBox &sb = boxes_[order[oi]].sb[fr];
+                nw = std::max(nw, sb.ntj);
+                fuse = fuse && stream_fused(sb);
+            }
+            G.sab_nw[fr] = fuse ? nw : 0;
             for (int oi = G.act_off; oi < G.act_off + G.act_n; oi++) {
                 const BoxDesc &d = boxes_[order[oi]];
                 if (!stream_now_[order[oi]]) continue;
-                for (int w = 0; w < d.nwa[fr]; w += 4) sitems[0].push_back({order[oi], w});
-                for (int w = 0; w < d.nwb[fr]; w += 4) sitems[1].push_back({order[oi], w});
+                if (G.sab_nw[fr]) {
+                    const int nb = (d.sb[fr].nk + 1) / 2 * d.sb[fr].nseg;
+                    for (int b = 0; b < nb; b++) sitems[0].push_back({order[oi], b});
+                } else {
+                    for (int w = 0; w < d.nwa[fr]; w += 4) sitems[0].push_back({order[oi], w});
+                    for (int w = 0; w < d.nwb[fr]; w += 4) sitems[1].push_back({order[oi], w});
+                }
             }
             G.sa_n[fr] = (int)sitems[0].size() - G.sa_off[fr];
             G.sb_n[fr] = (int)sitems[1].size() - G.sb_off[fr];
@@ -973,7 +1007,17 @@ int Engine::enqueue_step()
                         e = launch_group(fr, G.K[fr], true, G.tile_adv_n[fr], G.lds_a[fr], S, d_boxes_,
                                          d_items_[fr] + G.item_off[fr], d_dt_, flip);
                     if (e != hipSuccess || G.sa_n[fr] == 0) continue;
-                    if (fr == 1) {
+                    if (G.sab_nw[fr]) {
+                        const dim3 grid(G.sa_n[fr]), block(64 * G.sab_nw[fr]);
+                        const WorkItem *it = d_sitems_[0] + G.sa_off[fr];
+#define MHD_SAB(FR, NW) hipLaunchKernelGGL((batch_stream_ab_kernel<FR, NW>), grid, block, 0, S, d_boxes_, it, d_dt_, flip)
+                        if (G.sab_nw[fr] <= 4) {
+                            if (fr == 1) MHD_SAB(1, 4); else MHD_SAB(2, 4);
+                        } else {
+                            if (fr == 1) MHD_SAB(1, 16); else MHD_SAB(2, 16);
+                        }
+#undef MHD_SAB
+                    } else if (fr == 1) {
                         hipLaunchKernelGGL(batch_stream_a_kernel<1>, dim3(G.sa_n[1]), dim3(256), 0, S, d_boxes_,
                                            d_sitems_[0] + G.sa_off[1], d_dt_, flip);
                         hipLaunchKernelGGL(batch_stream_b_kernel<1>, dim3(G.sb_n[1]), dim3(256), 0, S, d_boxes_,
