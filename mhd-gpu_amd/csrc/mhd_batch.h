@@ -47,6 +47,7 @@ struct BoxGroup {
     int lds_f[3], lds_a[3];  // dynamic LDS bytes of fluidx / advect per frame
     int sa_off[3], sa_n[3], sb_off[3], sb_n[3];  // block items of the streamed y/z advectbyzx
     int sab_nw[3];  // waves per block of the fused stream (its items in the sa list); 0: the two passes
+    int f8_off[3], f8_n[3], lds_f8[3];  // y/z fluidx in tiles of two vectors (8 pencils); f8_n 0: one-vector tiles
     int tile_adv_n[3];  // leading items of item_off[fr] whose boxes run the tile advectbyzx
     long cells;
 };
@@ -131,6 +132,8 @@ private:
     WorkItem *d_items_[3] = {nullptr, nullptr, nullptr};
     WorkItem *d_sitems_[2] = {nullptr, nullptr};  // streamed advectbyzx blocks: pass A, pass B (both frames)
     int cap_sitems_[2] = {0, 0};
+    WorkItem *d_f8items_ = nullptr;  // y/z fluidx tiles of 8 pencils (both frames)
+    int cap_f8items_ = 0;
     CflItem *d_cfl_ = nullptr;
     int nitems_[3] = {0, 0, 0}, ncfl_ = 0, cap_items_[3] = {0, 0, 0}, cap_cfl_ = 0;
     std::vector<BoxGroup> groups_;

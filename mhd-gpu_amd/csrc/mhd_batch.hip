@@ -195,6 +195,40 @@ __global__ void __launch_bounds__(256) batch_stream_b_kernel(const BoxDesc *__re
                                    wave);
 }
 
+// y/z fluidx in tiles of two vectors (schedule 3): one block of 8 waves per
+// item (box, tile of 8 pencils), exact K
+template <int FR, int K>
+__global__ void __launch_bounds__(64 * 2 * kBW)
+    batch_fluidx_yz_kernel(const BoxDesc *__restrict__ boxes, const WorkItem *__restrict__ items,
+                           const float *__restrict__ dt, int flip)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const WorkItem wi = items[xcd_tile(blockIdx.x, gridDim.x)];
+    const BoxDesc &bd = boxes[wi.box];
+    const int ntj = (bd.fr[FR].f.nj + 2 * kBW - 1) / (2 * kBW);
+    fluidx_soa_tile<float, K, 2 * kBW, FR>(bd.u, flip ? bd.b2 : bd.b, bd.fr[FR], dt + wi.box, wi.tile, ntj,
+                                           reinterpret_cast<float *>(smem_raw));
+}
+
+static const void *fluidx_yz_kernel(int fr, int K)
+{
+#define MHD_FY(F)                                                                                                    \
+    if (fr == F) switch (K) {                                                                                        \
+        case 1: return (const void *)batch_fluidx_yz_kernel<F, 1>;                                                   \
+        case 2: return (const void *)batch_fluidx_yz_kernel<F, 2>;                                                   \
+        case 3: return (const void *)batch_fluidx_yz_kernel<F, 3>;                                                   \
+        case 4: return (const void *)batch_fluidx_yz_kernel<F, 4>;                                                   \
+        case 5: return (const void *)batch_fluidx_yz_kernel<F, 5>;                                                   \
+        case 6: return (const void *)batch_fluidx_yz_kernel<F, 6>;                                                   \
+        case 8: return (const void *)batch_fluidx_yz_kernel<F, 8>;                                                   \
+        default: return nullptr;                                                                                     \
+        }
+    MHD_FY(1)
+    MHD_FY(2)
+#undef MHD_FY
+    return nullptr;
+}
+
 // Fused y/z stream (schedule 3): one block per item (box, plane pair and
 // segment), sized for the group's widest box; the waves beyond a narrower
 // box's pencil tiles end at once (a barrier waits only for running waves)
@@ -396,7 +430,7 @@ Engine::~Engine()
     for (float *x : xbuf_)
         if (x) (void)hipFree(x);
     void *ps[] = {d_boxes_, d_dt_, d_t_, d_tf_, d_dts_, d_t64_, d_cmax_, d_iter_, d_active_, d_ctr_, d_part_,
-                  d_items_[0], d_items_[1], d_items_[2], d_cfl_, d_sitems_[0], d_sitems_[1]};
+                  d_items_[0], d_items_[1], d_items_[2], d_cfl_, d_sitems_[0], d_sitems_[1], d_f8items_};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (ev0_) (void)hipEventDestroy(ev0_);
@@ -459,10 +493,14 @@ int Engine::init()
                 if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute");
             }
     for (int fr = 0; fr < 3 && e == hipSuccess; fr++)
-        for (int K : {1, 2, 3, 4, 5, 6, 8})
+        for (int K : {1, 2, 3, 4, 5, 6, 8}) {
             for (int adv = 0; adv < 2 && e == hipSuccess; adv++)
                 e = hipFuncSetAttribute(group_kernel(fr, K, adv), hipFuncAttributeMaxDynamicSharedMemorySize,
                                         128 * 1024);
+            if (fr > 0 && e == hipSuccess)
+                e = hipFuncSetAttribute(fluidx_yz_kernel(fr, K), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        160 * 1024);
+        }
     if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute");
     // side streams (default priority) and the fork/join events of the
     // grouped schedule's step graph
@@ -927,6 +965,33 @@ int Engine::build()
             G.sb_n[fr] = (int)sitems[1].size() - G.sb_off[fr];
         }
     }
+    // y/z fluidx of a group in tiles of 8 pencils where two such tiles of its
+    // longest pencil fit a CU's LDS (as in the single-box path)
+    std::vector<WorkItem> f8items;
+    for (size_t g = 0; g < groups_.size(); g++) {
+        BoxGroup &G = groups_[g];
+        for (int fr = 1; fr < 3; fr++) {
+            G.f8_off[fr] = (int)f8items.size();
+            int lds = 0;
+            for (int oi = G.act_off; oi < G.act_off + G.act_n; oi++)
+                lds = std::max(lds, (int)(4 * fluidx_soa_words<2 * kBW>(boxes_[order[oi]].fr[fr].f.ni)));
+            G.lds_f8[fr] = lds;
+            if (lds > 80 * 1024) continue;
+            for (int oi = G.act_off; oi < G.act_off + G.act_n; oi++) {
+                const Frame &f = boxes_[order[oi]].fr[fr].f;
+                const int nt = (f.nj + 2 * kBW - 1) / (2 * kBW) * f.nk;
+                for (int t = 0; t < nt; t++) f8items.push_back({order[oi], t});
+            }
+            G.f8_n[fr] = (int)f8items.size() - G.f8_off[fr];
+        }
+    }
+    {
+        const int n = (int)f8items.size();
+        e = ensure_cap(&d_f8items_, &cap_f8items_, std::max(n, 1));
+        if (e == hipSuccess && n)
+            e = hipMemcpyAsync(d_f8items_, f8items.data(), n * sizeof(WorkItem), hipMemcpyHostToDevice, st_);
+        if (e != hipSuccess) return hip_fail(e, "engine build items");
+    }
     for (int p = 0; p < 2; p++) {
         const int n = (int)sitems[p].size();
         e = ensure_cap(&d_sitems_[p], &cap_sitems_[p], std::max(n, 1));
@@ -1029,6 +1094,12 @@ int Engine::enqueue_step()
                                            d_sitems_[1] + G.sb_off[2], d_dt_, flip);
                     }
                     e = hipGetLastError();
+                } else if (!adv && fr > 0 && G.f8_n[fr] > 0) {
+                    const void *k = fluidx_yz_kernel(fr, G.K[fr]);
+                    const WorkItem *it = d_f8items_ + G.f8_off[fr];
+                    void *args[] = {(void *)&d_boxes_, (void *)&it, (void *)&d_dt_, (void *)&flip};
+                    e = k ? hipLaunchKernel(k, dim3(G.f8_n[fr]), dim3(64 * 2 * kBW), args, G.lds_f8[fr], S)
+                          : hipErrorInvalidValue;
                 } else {
                     e = launch_group(fr, G.K[fr], adv, G.item_n[fr], adv ? G.lds_a[fr] : G.lds_f[fr], S, d_boxes_,
                                      d_items_[fr] + G.item_off[fr], d_dt_, flip);
