@@ -18,14 +18,14 @@
 
 namespace mhd {
 
-// y/z frames (LDS tiles)
-template <typename T, int K, int W, int FR>
+// y/z frames (LDS tiles); NF as in fluidx_soa_tile
+template <typename T, int K, int W, int FR, int NF = 0>
 __global__ void __launch_bounds__(64 * W)
     fluidx_soa_kernel(T *__restrict__ u, const T *__restrict__ b, SoaBox bx, const T *__restrict__ dtp, int ntj)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    fluidx_soa_tile<T, K, W, FR>(u, b, bx, dtp, xcd_tile(blockIdx.x, gridDim.x), ntj,
-                                 reinterpret_cast<T *>(smem_raw));
+    fluidx_soa_tile<T, K, W, FR, 0, NF>(u, b, bx, dtp, xcd_tile(blockIdx.x, gridDim.x), ntj,
+                                        reinterpret_cast<T *>(smem_raw));
 }
 
 template <typename T, int K, int W, int FR>
@@ -244,7 +244,8 @@ template <typename T, int K, int FR> static hipError_t soa_attr()
     static const hipError_t e = [] {
         if constexpr (FR != 0) {
             for (const void *k : {(const void *)fluidx_soa_kernel<T, K, W, FR>,
-                                  (const void *)fluidx_soa_kernel<T, K, 2 * W, FR>}) {
+                                  (const void *)fluidx_soa_kernel<T, K, 2 * W, FR>,
+                                  (const void *)fluidx_soa_kernel<T, K, 2 * W, FR, 64 * K>}) {
                 const hipError_t a = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
                 if (a != hipSuccess) return a;
             }
@@ -292,8 +293,13 @@ static hipError_t fluidx_soa_k(T *u, const T *b, const SoaBox &bx, const T *dt, 
         const size_t lds2 = sizeof(T) * fluidx_soa_words<W2>(bx.f.ni);
         if (yzw >= 2 && lds2 <= 80 * 1024) {
             const int ntj2 = (bx.f.nj + W2 - 1) / W2;
-            hipLaunchKernelGGL((fluidx_soa_kernel<T, K, W2, FR>), dim3(ntj2 * bx.f.nk), dim3(64 * W2), lds2, st, u, b,
-                               bx, dt, ntj2);
+            // a pencil that fills the wave: its length as a constant
+            if (bx.f.ni == 64 * K)
+                hipLaunchKernelGGL((fluidx_soa_kernel<T, K, W2, FR, 64 * K>), dim3(ntj2 * bx.f.nk), dim3(64 * W2),
+                                   lds2, st, u, b, bx, dt, ntj2);
+            else
+                hipLaunchKernelGGL((fluidx_soa_kernel<T, K, W2, FR>), dim3(ntj2 * bx.f.nk), dim3(64 * W2), lds2, st,
+                                   u, b, bx, dt, ntj2);
         } else {
             hipLaunchKernelGGL((fluidx_soa_kernel<T, K, W, FR>), grid, block, lds, st, u, b, bx, dt, ntj);
         }

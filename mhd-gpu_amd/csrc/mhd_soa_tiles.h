@@ -205,17 +205,20 @@ __device__ __forceinline__ void store_cells(T *__restrict__ p, const T (&o)[K])
 // new state (calcfl.c:45-56, face B / cell-averaged B of OpenMP/mhd.f90:
 // 112-120), the block's max written to cfl_part[tile] - the sweep is the
 // step's last, so that is the next step's CFL
-template <typename T, int K, int W, int FR, int CFLM = 0>
+// NF: the pencil length when the caller fixes it at compile time (n = 64K,
+// a full wave: the LDS offsets of the slots become immediates), else 0
+template <typename T, int K, int W, int FR, int CFLM = 0, int NF = 0>
 __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__restrict__ b, const SoaBox &bx,
                                                 const T *__restrict__ dtp, int tile, int ntj, T *smem,
                                                 T *__restrict__ cfl_part = nullptr)
 {
     static_assert(CFLM == 0 || FR == 0, "the CFL is fused into the x sweep only");
+    static_assert(NF == 0 || NF == 64 * K, "a fixed pencil length fills the wave");
     constexpr int VE = Vec16<T>::n;
     static_assert(W == VE || (FR != 0 && W == 2 * VE), "tile width is one 16-byte vector (two in the y/z frames)");
     const Frame &f = bx.f;
     const TileCtx t = tile_ctx<W, VE>(bx, tile, ntj);
-    const int n = t.n, ns = t.ns, L = n / K;
+    const int n = NF ? NF : t.n, ns = t.ns, L = n / K;
     const long N = bx.N;
     T *su = smem, *sb = su + 5 * W * n, *sb2n = sb + 3 * W * n, *sbk = sb2n + n;
     const int jn = wrapn(t.j0 + ns, f.nj);  // pencil after the tile (jp of the last one)
