@@ -55,8 +55,8 @@ __global__ void __launch_bounds__(64 * W)
                     const T *__restrict__ dtp, int ntj)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    advect_x_tile<T, K, W, VEC>(u, bin, bout, bx, dtp, xcd_tile(blockIdx.x, gridDim.x), ntj,
-                                reinterpret_cast<T *>(smem_raw));
+    advect_x2_tile<T, K, W, VEC>(u, bin, bout, bx, dtp, xcd_tile(blockIdx.x, gridDim.x), ntj,
+                                 reinterpret_cast<T *>(smem_raw));
 }
 
 // column-streaming advectbyzx of the y and z frames (mhd_soa_stream.h):
@@ -313,15 +313,17 @@ static hipError_t advect_soa_k(const T *u, const T *bin, T *bout, const SoaBox &
     constexpr int W = soa_w<T>();
     if constexpr (FR == 0) {
         // x frame: direct loads, LDS only for the fluxes
+        // tiles of W pencils in two planes (advect_x2_tile)
         const int ntj = (bx.f.nj + W - 1) / W;
-        const size_t lds = sizeof(T) * advect_x_words<W>(bx.f.ni);
+        const size_t lds = sizeof(T) * advect_x2_words<W>(bx.f.ni);
+        const dim3 grid(ntj * ((bx.f.nk + 1) / 2));
         constexpr bool kvec = K % (16 / sizeof(T)) == 0;
         if (kvec && bx.vec_ok)
-            hipLaunchKernelGGL((advect_x_kernel<T, K, W, kvec>), dim3(ntj * bx.f.nk), dim3(64 * W), lds, st, u, bin,
-                               bout, bx, dt, ntj);
+            hipLaunchKernelGGL((advect_x_kernel<T, K, W, kvec>), grid, dim3(64 * W), lds, st, u, bin, bout, bx, dt,
+                               ntj);
         else
-            hipLaunchKernelGGL((advect_x_kernel<T, K, W, false>), dim3(ntj * bx.f.nk), dim3(64 * W), lds, st, u, bin,
-                               bout, bx, dt, ntj);
+            hipLaunchKernelGGL((advect_x_kernel<T, K, W, false>), grid, dim3(64 * W), lds, st, u, bin, bout, bx, dt,
+                               ntj);
         return hipGetLastError();
     }
     hipError_t e = soa_attr<T, K, FR>();
