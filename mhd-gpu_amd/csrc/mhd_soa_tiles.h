@@ -979,7 +979,10 @@ __device__ __forceinline__ void advect_x2_tile(const T *__restrict__ u, const T 
     constexpr int FR = 0;
     const Frame &f = bx.f;
     const int n = f.ni, L = n / K;
-    const int kq = tile / ntj, j0 = (tile - kq * ntj) * W, ns = min(W, f.nj - j0);
+    // plane pairs fastest: neighbouring blocks share the planes k-1 and k+2
+    // they read (the u planes of the k-faces) in L2
+    const int nk2 = (f.nk + 1) / 2, jt = tile / nk2, kq = tile - jt * nk2;
+    const int j0 = jt * W, ns = min(W, f.nj - j0);
     const int k = 2 * kq, P = k + 1 < f.nk ? 2 : 1;
     // planes k-1 .. k+2, periodic
     long kpl[4];
