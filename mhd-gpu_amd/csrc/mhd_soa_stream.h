@@ -72,6 +72,28 @@ template <typename T> struct TvdbStream {
         // tvdb.c:87 b = b - (flux - flux[i-1]) at t-4
         bnew = bw4 - (fl4 - fl5);
         F = fl5;
+        shift(vr0, vg1, bt, up2, f12, g2);
+        fl5 = fl4;
+    }
+
+    // A push that only fills the window: the first rows of a segment, whose
+    // limited flux nothing reads (it takes the limiter from the eighth push
+    // on, whose flux the first emitted row uses).  Leaves fl5 undefined.
+    __device__ __forceinline__ void push_fill(T ma, T mb, T ra, T rb, T bt, T dt)
+    {
+        const T vr0 = face_vel(ma, mb, ra, rb);
+        const T vg1 = (T)(((double)(vr2 + vr0) + 2.0 * (double)vr1) / 4.0);
+        const bool up2 = (T)0.5 * vg2 + (T)0.5 * vg1 > (T)0;
+        const T bv1 = bw1 * vg1;
+        const T bv2 = bw2 * vg2;
+        const T f12 = up2 ? bv2 : bv1;
+        const T b12 = (T)((double)bw2 - (double)((f12 - f13) * dt) / 2.0);
+        const T g2 = vg2 * b12;
+        shift(vr0, vg1, bt, up2, f12, g2);
+    }
+
+    __device__ __forceinline__ void shift(T vr0, T vg1, T bt, bool up2, T f12, T g2)
+    {
         vr2 = vr1;
         vr1 = vr0;
         vg2 = vg1;
@@ -85,7 +107,6 @@ template <typename T> struct TvdbStream {
         g5 = g4;
         g4 = g3;
         g3 = g2;
-        fl5 = fl4;
     }
 };
 
@@ -130,7 +151,10 @@ __device__ __forceinline__ void advect_stream_a(const T *__restrict__ u, const T
             bt = b2[ro + jj];
         }
         T bnew, F;
-        st.push(cma, cmb, cra, crb, cbt, dt, bnew, F);
+        if (t < s0 + 3)
+            st.push_fill(cma, cmb, cra, crb, cbt, dt);
+        else
+            st.push(cma, cmb, cra, crb, cbt, dt, bnew, F);
         if (t >= s0 + 4 && valid) {
             const long oo = (long)(t - 4) * sb.si + kb + jj;
             b2o[oo] = bnew;
@@ -196,12 +220,18 @@ __device__ __forceinline__ void advect_stream_b(const T *__restrict__ u, const T
         const Row q = nx;
         if (t + 1 < s1 + 4) load_row(t + 1, nx);
         T b3n0, F0, b3n1, F1;
-        sa.push(q.mm, q.m0, q.rm, q.r0, q.b30, dt, b3n0, F0);
-        sbk.push(q.m0, q.m1, q.r0, q.r1, q.b31, dt, b3n1, F1);
         T F2 = 0;
-        if (two) {
-            T b3x;
-            sc.push(q.m1, q.m2, q.r1, q.r2, q.b32, dt, b3x, F2);
+        if (t < s0 + 3) {
+            sa.push_fill(q.mm, q.m0, q.rm, q.r0, q.b30, dt);
+            sbk.push_fill(q.m0, q.m1, q.r0, q.r1, q.b31, dt);
+            if (two) sc.push_fill(q.m1, q.m2, q.r1, q.r2, q.b32, dt);
+        } else {
+            sa.push(q.mm, q.m0, q.rm, q.r0, q.b30, dt, b3n0, F0);
+            sbk.push(q.m0, q.m1, q.r0, q.r1, q.b31, dt, b3n1, F1);
+            if (two) {
+                T b3x;
+                sc.push(q.m1, q.m2, q.r1, q.r2, q.b32, dt, b3x, F2);
+            }
         }
         if (t >= s0 + 4 && valid) {
             const long row = (long)(t - 4) * sb.si;
@@ -306,10 +336,22 @@ __device__ __forceinline__ void advect_stream_ab(const T *__restrict__ u, const 
         fb1.push(q.m0, q.m1, q.r0, q.r1, q.b31, dt, b3n1, f1);
         fb2.push(q.m1, q.m2, q.r1, q.r2, q.b32, dt, b3x, f2);
     };
-    // the first eight rows fill the windows
-    for (int t = s0 - 4; t < s0 + 4; t++) {
+    // the first eight rows fill the windows (the limiter from the eighth on)
+    auto fill_row = [&](const Row &q) {
+        a0.push_fill(q.mm0, q.m0, q.rm0, q.r0, q.b20, dt);
+        a1.push_fill(q.mm1, q.m1, q.rm1, q.r1, q.b21, dt);
+        fb0.push_fill(q.mk, q.m0, q.rk, q.r0, q.b30, dt);
+        fb1.push_fill(q.m0, q.m1, q.r0, q.r1, q.b31, dt);
+        fb2.push_fill(q.m1, q.m2, q.r1, q.r2, q.b32, dt);
+    };
+    for (int t = s0 - 4; t < s0 + 3; t++) {
         const Row q = nx;
         load_row(t + 1, nx);
+        fill_row(q);
+    }
+    {
+        const Row q = nx;
+        load_row(s0 + 4, nx);
         push_row(q);
     }
     // b1 of the output row, loaded one row ahead (rows wrap, so the load
