@@ -206,8 +206,13 @@ __global__ void __launch_bounds__(64 * 2 * kBW)
     const WorkItem wi = items[xcd_tile(blockIdx.x, gridDim.x)];
     const BoxDesc &bd = boxes[wi.box];
     const int ntj = (bd.fr[FR].f.nj + 2 * kBW - 1) / (2 * kBW);
-    fluidx_soa_tile<float, K, 2 * kBW, FR>(bd.u, flip ? bd.b2 : bd.b, bd.fr[FR], dt + wi.box, wi.tile, ntj,
-                                           reinterpret_cast<float *>(smem_raw));
+    float *smem = reinterpret_cast<float *>(smem_raw);
+    // a pencil that fills the wave: its length as a constant (uniform branch)
+    if (bd.fr[FR].f.ni == 64 * K)
+        fluidx_soa_tile<float, K, 2 * kBW, FR, 0, 64 * K>(bd.u, flip ? bd.b2 : bd.b, bd.fr[FR], dt + wi.box, wi.tile,
+                                                          ntj, smem);
+    else
+        fluidx_soa_tile<float, K, 2 * kBW, FR>(bd.u, flip ? bd.b2 : bd.b, bd.fr[FR], dt + wi.box, wi.tile, ntj, smem);
 }
 
 static const void *fluidx_yz_kernel(int fr, int K)
