@@ -138,6 +138,13 @@ private:
     int nitems_[3] = {0, 0, 0}, ncfl_ = 0, cap_items_[3] = {0, 0, 0}, cap_cfl_ = 0;
     std::vector<BoxGroup> groups_;
     std::vector<char> stream_now_;  // per slot: the y/z advectbyzx streams in the current build
+    // per slot, schedule 3: the CFL accumulator holds the CFL of the slot's
+    // current state (the last sweep of every step refreshes it; a set state,
+    // a clock reset or a schedule change make it stale, and step() then runs
+    // the standalone CFL over the slot's items cfl_slot_off_ .. + cfl_slot_n_)
+    std::vector<char> cfl_fresh_;
+    std::vector<int> cfl_slot_off_, cfl_slot_n_;
+    int prime_cfl();
     static constexpr int kAux = 4;  // side streams of the grouped schedule
     hipStream_t aux_[kAux] = {};
     hipEvent_t evfork_ = nullptr, evjoin_[kAux] = {};

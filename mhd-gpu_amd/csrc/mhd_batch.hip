@@ -195,6 +195,44 @@ __global__ void __launch_bounds__(256) batch_stream_b_kernel(const BoxDesc *__re
                                    wave);
 }
 
+// The step's last sweep (fluidx, x frame) with the next step's CFL
+// (schedule 3): the x-frame group kernel's body, each block adding its max
+// CFL speed to the box's accumulator, which the next step's dt kernel reads.
+// CFLM 1 / 2: face / cell-averaged B.
+template <int K, int CFLM>
+__global__ void __launch_bounds__(64 * kBW)
+    batch_fluidx_x_cfl_kernel(const BoxDesc *__restrict__ boxes, const WorkItem *__restrict__ items,
+                              const float *__restrict__ dt, int flip, unsigned long long *__restrict__ cmax)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float *smem = reinterpret_cast<float *>(smem_raw);
+    const WorkItem wi = items[xcd_tile(blockIdx.x, gridDim.x)];
+    const BoxDesc &bd = boxes[wi.box];
+    const float *bcur = flip ? bd.b2 : bd.b;
+    if (K % 4 == 0 && bd.fr[0].vec_ok)
+        fluidx_x_tile<float, K, kBW, K % 4 == 0, CFLM>(bd.u, bcur, bd.fr[0], dt + wi.box, wi.tile, bd.ntj[0], smem,
+                                                        nullptr, cmax + wi.box);
+    else
+        fluidx_x_tile<float, K, kBW, false, CFLM>(bd.u, bcur, bd.fr[0], dt + wi.box, wi.tile, bd.ntj[0], smem,
+                                                   nullptr, cmax + wi.box);
+}
+
+static const void *fluidx_x_cfl_kernel(int K, int cflm)
+{
+#define MHD_FC(KK) return cflm == 2 ? (const void *)batch_fluidx_x_cfl_kernel<KK, 2> : (const void *)batch_fluidx_x_cfl_kernel<KK, 1>
+    switch (K) {
+    case 1: MHD_FC(1);
+    case 2: MHD_FC(2);
+    case 3: MHD_FC(3);
+    case 4: MHD_FC(4);
+    case 5: MHD_FC(5);
+    case 6: MHD_FC(6);
+    case 8: MHD_FC(8);
+    default: return nullptr;
+    }
+#undef MHD_FC
+}
+
 // y/z fluidx in tiles of two vectors (schedule 3): one block of 8 waves per
 // item (box, tile of 8 pencils), exact K
 template <int FR, int K>
@@ -479,6 +517,9 @@ int Engine::init()
     boxes_.assign(B, BoxDesc());
     used_.assign(B, false);
     tf_.assign(B, INFINITY);
+    cfl_fresh_.assign(B, 0);
+    cfl_slot_off_.assign(B, 0);
+    cfl_slot_n_.assign(B, 0);
     hipDeviceProp_t prop;
     int dev = 0;
     e = hipGetDevice(&dev);
@@ -505,6 +546,9 @@ int Engine::init()
             if (fr > 0 && e == hipSuccess)
                 e = hipFuncSetAttribute(fluidx_yz_kernel(fr, K), hipFuncAttributeMaxDynamicSharedMemorySize,
                                         160 * 1024);
+            for (int cflm = 1; fr == 0 && cflm <= 2 && e == hipSuccess; cflm++)
+                e = hipFuncSetAttribute(fluidx_x_cfl_kernel(K, cflm), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        128 * 1024);
         }
     if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute");
     // side streams (default priority) and the fork/join events of the
@@ -673,6 +717,7 @@ int Engine::transfer(int slot, void *h_u, void *h_b, int to_device, hipStream_t 
         float *d;
         int nc;
     } ps[2] = {{h_u, d.u, 5}, {h_b, d.b, 3}};
+    if (to_device) cfl_fresh_[slot] = 0;
     for (auto &p : ps) {
         if (e != hipSuccess) break;
         if (to_device) {
@@ -749,6 +794,7 @@ int Engine::reset_clock(int slot, double tf)
 int Engine::reset_clock_on(int slot, double tf, hipStream_t st)
 {
     tf_[slot] = tf;
+    cfl_fresh_[slot] = 0;
     const float tff = (float)tf;
     hipError_t e = hipMemcpyAsync(d_tf_ + slot, &tff, 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemsetAsync(d_t_ + slot, 0, 4, st);
@@ -807,6 +853,8 @@ int Engine::set_schedule(int bpc, int policy)
     bpc_ = bpc;
     policy_ = policy;
     dirty_ = true;
+    // the other schedules do not keep the CFL accumulators current
+    for (auto &f : cfl_fresh_) f = 0;
     return MHD_OK;
 }
 
@@ -1013,7 +1061,9 @@ int Engine::build()
             const int s = order[oi];
             if (gnext < groups_.size() && (int)oi == groups_[gnext].act_off) groups_[gnext++].cfl_off = (int)cfl.size();
             const long units = cfl_units<float>(boxes_[s].g);
+            cfl_slot_off_[s] = (int)cfl.size();
             for (long e = 0; e < units; e += 1024) cfl.push_back({s, (int)e, (int)std::min(1024L, units - e)});
+            cfl_slot_n_[s] = (int)cfl.size() - cfl_slot_off_[s];
         }
         for (size_t g = 0; g < groups_.size(); g++)
             groups_[g].cfl_n = (g + 1 < groups_.size() ? groups_[g + 1].cfl_off : (int)cfl.size()) - groups_[g].cfl_off;
@@ -1055,13 +1105,9 @@ int Engine::enqueue_step()
         for (int g = 0; g < ng && e == hipSuccess; g++) {
             const BoxGroup &G = groups_[g];
             hipStream_t S = aux_[g % kAux];
-            if (avg_)
-                hipLaunchKernelGGL((batch_cfl_kernel<true>), dim3(G.cfl_n), dim3(256), 0, S, d_boxes_, d_cfl_ + G.cfl_off,
-                                   d_cmax_);
-            else
-                hipLaunchKernelGGL((batch_cfl_kernel<false>), dim3(G.cfl_n), dim3(256), 0, S, d_boxes_,
-                                   d_cfl_ + G.cfl_off, d_cmax_);
-            e = hipGetLastError();
+            // the CFL of the state the step starts from is already in the
+            // accumulator: the previous step's last sweep put it there (or the
+            // standalone CFL run by step() after the state was set)
             if (e == hipSuccess) {
                 hipLaunchKernelGGL(batch_dt_kernel, dim3(1), dim3(256), 0, S, d_active_ + G.act_off, G.act_n, d_cmax_,
                                    d_dt_, d_t_, d_t64_, d_tf_, d_dts_, cap_, d_iter_, d_ctr_, 0);
@@ -1099,6 +1145,13 @@ int Engine::enqueue_step()
                                            d_sitems_[1] + G.sb_off[2], d_dt_, flip);
                     }
                     e = hipGetLastError();
+                } else if (sw == 11) {
+                    // the last sweep (fluidx, x frame) also forms the next step's CFL
+                    const void *k = fluidx_x_cfl_kernel(G.K[0], avg_ ? 2 : 1);
+                    const WorkItem *it = d_items_[0] + G.item_off[0];
+                    void *args[] = {(void *)&d_boxes_, (void *)&it, (void *)&d_dt_, (void *)&flip, (void *)&d_cmax_};
+                    e = k ? hipLaunchKernel(k, dim3(G.item_n[0]), dim3(64 * kBW), args, G.lds_f[0], S)
+                          : hipErrorInvalidValue;
                 } else if (!adv && fr > 0 && G.f8_n[fr] > 0) {
                     const void *k = fluidx_yz_kernel(fr, G.K[fr]);
                     const WorkItem *it = d_f8items_ + G.f8_off[fr];
@@ -1182,11 +1235,37 @@ int Engine::prepare()
     return exec_ ? MHD_OK : capture();
 }
 
+// Schedule 3: the standalone CFL, on the compute stream before the step's
+// replay, for the active slots whose accumulator is stale (the accumulator is
+// cleared first; outside any captured graph)
+int Engine::prime_cfl()
+{
+    if (policy_ != 3) return MHD_OK;
+    hipError_t e = hipSuccess;
+    for (int s : active_) {
+        if (cfl_fresh_[s] || e != hipSuccess) continue;
+        e = hipMemsetAsync(d_cmax_ + s, 0, 8, st_);
+        if (e == hipSuccess && cfl_slot_n_[s] > 0) {
+            if (avg_)
+                hipLaunchKernelGGL((batch_cfl_kernel<true>), dim3(cfl_slot_n_[s]), dim3(256), 0, st_, d_boxes_,
+                                   d_cfl_ + cfl_slot_off_[s], d_cmax_);
+            else
+                hipLaunchKernelGGL((batch_cfl_kernel<false>), dim3(cfl_slot_n_[s]), dim3(256), 0, st_, d_boxes_,
+                                   d_cfl_ + cfl_slot_off_[s], d_cmax_);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) cfl_fresh_[s] = 1;
+    }
+    return e == hipSuccess ? MHD_OK : hip_fail(e, "engine CFL");
+}
+
 int Engine::step(int nsteps)
 {
     if (nsteps < 0) return fail(MHD_ERR_ARG, "step: nsteps < 0");
     if (active_.empty() || nsteps == 0) return MHD_OK;
     int rc = prepare();
+    if (rc) return rc;
+    rc = prime_cfl();
     if (rc) return rc;
     for (int i = 0; i < nsteps; i++) {
         hipError_t e = hipGraphLaunch(exec_, st_);
@@ -1275,6 +1354,7 @@ int Engine::restore_all(std::vector<float *> &saved)
     }
     if (e == hipSuccess) e = hipMemsetAsync(d_cmax_, 0, B * 8, st_);
     if (e == hipSuccess) e = hipStreamSynchronize(st_);
+    for (auto &f : cfl_fresh_) f = 0;
     for (float *p : saved) (void)hipFree(p);
     saved.clear();
     return e == hipSuccess ? MHD_OK : hip_fail(e, "restore state");

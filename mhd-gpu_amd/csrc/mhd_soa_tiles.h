@@ -850,7 +850,7 @@ template <int W> __host__ __device__ constexpr int fluidx_x_words(int n, int cfl
 template <typename T, int K, int W, bool VEC, int CFLM>
 __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__restrict__ b, const SoaBox &bx,
                                               const T *__restrict__ dtp, int tile, int ntj, T *su,
-                                              T *__restrict__ cfl_part)
+                                              T *__restrict__ cfl_part, unsigned long long *cmax = nullptr)
 {
     const Frame &f = bx.f;
     const TileCtx t = tile_ctx<W>(bx, tile, ntj);
@@ -955,7 +955,13 @@ __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__rest
         if (threadIdx.x == 0) {
             T m = cfl_red_slot<T>()[0];
             for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = fmax(m, cfl_red_slot<T>()[i]);
-            cfl_part[tile] = m;
+            // the block max: into the tile's slot, or (FP32, cmax given) into
+            // the box's accumulator as float bits (all speeds >= 0: bit
+            // order is value order)
+            if (sizeof(T) == 4 && cmax)
+                atomicMax(cmax, (unsigned long long)__float_as_uint((float)m));
+            else
+                cfl_part[tile] = m;
         }
     }
 }
