@@ -175,8 +175,9 @@ int mhd_ctx_profile_step(mhd_ctx *ctx, int reps, float *ms_out13);
  * exactly as a lone context would (results are bit-identical to
  * mhd_ctx_step on the same box).  One step of the batch is captured in a
  * hipGraph: under the default schedule the boxes are grouped by their
- * cells-per-lane K, and each group runs one CFL launch, one dt launch and
- * 12 sweep kernels over all its boxes, the groups concurrently.  FP32.
+ * cells-per-lane K, and each group runs one dt launch and 12 sweep kernels
+ * over all its boxes (the last sweep also forms the next step's CFL), the
+ * groups concurrently.  FP32.
  * dims holds nx,ny,nz of each box. */
 typedef struct mhd_batch mhd_batch;
 int mhd_batch_create(mhd_batch **bt, int nbox, const int *dims, int avg_b);
