@@ -48,6 +48,7 @@ struct BoxGroup {
     int sa_off[3], sa_n[3], sb_off[3], sb_n[3];  // block items of the streamed y/z advectbyzx
     int sab_nw[3];  // waves per block of the fused stream (its items in the sa list); 0: the two passes
     int f8_off[3], f8_n[3], lds_f8[3];  // y/z fluidx in tiles of two vectors (8 pencils); f8_n 0: one-vector tiles
+    int x2_off, x2_n, lds_x2;           // x-frame advectbyzx in tiles of two planes (in the f8 item list)
     int tile_adv_n[3];  // leading items of item_off[fr] whose boxes run the tile advectbyzx
     long cells;
 };

@@ -233,6 +233,39 @@ static const void *fluidx_x_cfl_kernel(int K, int cflm)
 #undef MHD_FC
 }
 
+// x-frame advectbyzx in tiles of two planes (schedule 3): one block per item
+// (box, tile of kBW pencils in a plane pair), exact K
+template <int K>
+__global__ void __launch_bounds__(64 * kBW)
+    batch_advect_x2_kernel(const BoxDesc *__restrict__ boxes, const WorkItem *__restrict__ items,
+                           const float *__restrict__ dt, int flip)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float *smem = reinterpret_cast<float *>(smem_raw);
+    const WorkItem wi = items[xcd_tile(blockIdx.x, gridDim.x)];
+    const BoxDesc &bd = boxes[wi.box];
+    const float *bcur = flip ? bd.b2 : bd.b;
+    float *bnext = flip ? bd.b : bd.b2;
+    if (K % 4 == 0 && bd.fr[0].vec_ok)
+        advect_x2_tile<float, K, kBW, K % 4 == 0>(bd.u, bcur, bnext, bd.fr[0], dt + wi.box, wi.tile, bd.ntj[0], smem);
+    else
+        advect_x2_tile<float, K, kBW, false>(bd.u, bcur, bnext, bd.fr[0], dt + wi.box, wi.tile, bd.ntj[0], smem);
+}
+
+static const void *advect_x2_kernel(int K)
+{
+    switch (K) {
+    case 1: return (const void *)batch_advect_x2_kernel<1>;
+    case 2: return (const void *)batch_advect_x2_kernel<2>;
+    case 3: return (const void *)batch_advect_x2_kernel<3>;
+    case 4: return (const void *)batch_advect_x2_kernel<4>;
+    case 5: return (const void *)batch_advect_x2_kernel<5>;
+    case 6: return (const void *)batch_advect_x2_kernel<6>;
+    case 8: return (const void *)batch_advect_x2_kernel<8>;
+    default: return nullptr;
+    }
+}
+
 // y/z fluidx in tiles of two vectors (schedule 3): one block of 8 waves per
 // item (box, tile of 8 pencils), exact K
 template <int FR, int K>
@@ -546,6 +579,8 @@ int Engine::init()
             if (fr > 0 && e == hipSuccess)
                 e = hipFuncSetAttribute(fluidx_yz_kernel(fr, K), hipFuncAttributeMaxDynamicSharedMemorySize,
                                         160 * 1024);
+            if (fr == 0 && e == hipSuccess)
+                e = hipFuncSetAttribute(advect_x2_kernel(K), hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024);
             for (int cflm = 1; fr == 0 && cflm <= 2 && e == hipSuccess; cflm++)
                 e = hipFuncSetAttribute(fluidx_x_cfl_kernel(K, cflm), hipFuncAttributeMaxDynamicSharedMemorySize,
                                         128 * 1024);
@@ -1037,6 +1072,16 @@ int Engine::build()
             }
             G.f8_n[fr] = (int)f8items.size() - G.f8_off[fr];
         }
+        // x-frame advectbyzx tiles of two planes
+        G.x2_off = (int)f8items.size();
+        G.lds_x2 = 0;
+        for (int oi = G.act_off; oi < G.act_off + G.act_n; oi++) {
+            const BoxDesc &d = boxes_[order[oi]];
+            G.lds_x2 = std::max(G.lds_x2, (int)(4 * advect_x2_words<kBW>(d.fr[0].f.ni)));
+            const int nt = d.ntj[0] * ((d.fr[0].f.nk + 1) / 2);
+            for (int t = 0; t < nt; t++) f8items.push_back({order[oi], t});
+        }
+        G.x2_n = (int)f8items.size() - G.x2_off;
     }
     {
         const int n = (int)f8items.size();
@@ -1145,6 +1190,11 @@ int Engine::enqueue_step()
                                            d_sitems_[1] + G.sb_off[2], d_dt_, flip);
                     }
                     e = hipGetLastError();
+                } else if (adv && fr == 0 && G.x2_n > 0) {
+                    const void *k = advect_x2_kernel(G.K[0]);
+                    const WorkItem *it = d_f8items_ + G.x2_off;
+                    void *args[] = {(void *)&d_boxes_, (void *)&it, (void *)&d_dt_, (void *)&flip};
+                    e = k ? hipLaunchKernel(k, dim3(G.x2_n), dim3(64 * kBW), args, G.lds_x2, S) : hipErrorInvalidValue;
                 } else if (sw == 11) {
                     // the last sweep (fluidx, x frame) also forms the next step's CFL
                     const void *k = fluidx_x_cfl_kernel(G.K[0], avg_ ? 2 : 1);
