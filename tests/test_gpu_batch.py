@@ -85,3 +85,27 @@ def test_batch_streamed_and_tiled_boxes_equal_alone(M):
         assert bitwise(bt.dts(i), da)
         assert bitwise(ug, ua) and bitwise(bg, ba)
     bt.close()
+
+
+def test_batch_fused_stream_mixed_widths_equal_alone(M):
+    # one group (cells per lane 8, 2, 2) whose boxes are 8 and 7 pencil tiles
+    # wide: the fused y/z stream runs blocks of 8 waves, and for the narrower
+    # box the eighth wave ends at once; then a state is set again between
+    # steps (the CFL accumulator of that box must be formed anew)
+    dims = [(512, 96, 96), (448, 96, 96)]
+    states = [random_state(*d, seed=970 + i, amp=0.3) for i, d in enumerate(dims)]
+    bt = M.Batch(dims)
+    bt.set_schedule(0, 3)
+    for i, (u, b) in enumerate(states):
+        bt.set_state(i, u, b)
+    bt.step(2)
+    u1, b1 = bt.get_state(1)
+    bt.set_state(1, u1, b1)
+    bt.step(1)
+    for i, d in enumerate(dims):
+        print("box", i, d, flush=True)
+        ug, bg = bt.get_state(i)
+        ua, ba, da = alone(M, d, *states[i], 3)
+        assert bitwise(bt.dts(i), da)
+        assert bitwise(ug, ua) and bitwise(bg, ba)
+    bt.close()
