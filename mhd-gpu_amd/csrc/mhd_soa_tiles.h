@@ -985,9 +985,21 @@ __device__ __forceinline__ void advect_x2_tile(const T *__restrict__ u, const T 
     constexpr int FR = 0;
     const Frame &f = bx.f;
     const int n = f.ni, L = n / K;
-    // plane pairs fastest: neighbouring blocks share the planes k-1 and k+2
-    // they read (the u planes of the k-faces) in L2
-    const int nk2 = (f.nk + 1) / 2, jt = tile / nk2, kq = tile - jt * nk2;
+    // tiles in groups of 8 plane pairs by all pencil tiles, the plane pairs
+    // fastest: a tile's neighbours along j (pencil j0-1, j0+W) and along k
+    // (planes k-1, k+2) are mostly blocks close in launch order, so their
+    // rows come from L2
+    const int nk2 = (f.nk + 1) / 2, nfull = nk2 >> 3, ing = ntj << 3;
+    int jt, kq;
+    if (tile < nfull * ing) {
+        const int kg = tile / ing, r = tile - kg * ing;
+        jt = r >> 3;
+        kq = (kg << 3) + (r & 7);
+    } else {
+        const int rem = nk2 - (nfull << 3), r = tile - nfull * ing;
+        jt = r / rem;
+        kq = (nfull << 3) + (r - jt * rem);
+    }
     const int j0 = jt * W, ns = min(W, f.nj - j0);
     const int k = 2 * kq, P = k + 1 < f.nk ? 2 : 1;
     // planes k-1 .. k+2, periodic
