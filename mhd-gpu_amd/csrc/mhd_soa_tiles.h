@@ -201,6 +201,126 @@ __device__ __forceinline__ void store_cells(T *__restrict__ p, const T (&o)[K])
     }
 }
 
+// One 16-byte vector of row i of a y/z tile (h-th of the row's H vectors):
+// the 5 u and 3 b components at plane k, b3 at k+1 and, with the row's last
+// vector, b2 of the pencil after the tile
+template <typename T> struct YzRow {
+    V16<T> u[5], b[4];
+    T b2n;
+};
+
+template <typename T, int FR>
+__device__ __forceinline__ void yz_row_load(YzRow<T> &r, const T *__restrict__ u, const T *__restrict__ b,
+                                            const SoaBox &bx, const TileCtx &t, int jn, int i, int h, bool last)
+{
+    constexpr int VE = Vec16<T>::n;
+    const long N = bx.N, ro = (long)i * bx.f.si + t.j0 + h * VE;
+#pragma unroll
+    for (int c = 0; c < 5; c++) r.u[c] = *reinterpret_cast<const V16<T> *>(u + cu<FR>(c) * N + t.kb + ro);
+#pragma unroll
+    for (int c = 0; c < 3; c++) r.b[c] = *reinterpret_cast<const V16<T> *>(b + cb<FR>(c) * N + t.kb + ro);
+    r.b[3] = *reinterpret_cast<const V16<T> *>(b + Comp<FR>::b[2] * N + t.kpb + ro);
+    r.b2n = (T)0;
+    if (last) r.b2n = b[Comp<FR>::b[1] * N + t.kb + (long)i * bx.f.si + jn];
+}
+
+// the row vector's VE values into their slots [slot][component][cell]
+template <typename T>
+__device__ __forceinline__ void yz_row_scatter(const YzRow<T> &r, T *su, T *sb, T *sb2n, T *sbk, int n, int i, int h,
+                                               bool last)
+{
+    constexpr int VE = Vec16<T>::n;
+#pragma unroll
+    for (int v = 0; v < VE; v++) {
+        const int s = h * VE + v;
+#pragma unroll
+        for (int c = 0; c < 5; c++) su[(s * 5 + c) * n + i] = vget<T>(r.u[c], v);
+#pragma unroll
+        for (int c = 0; c < 3; c++) sb[(s * 3 + c) * n + i] = vget<T>(r.b[c], v);
+        sbk[s * n + i] = vget<T>(r.b[3], v);
+    }
+    if (last) sb2n[i] = r.b2n;
+}
+
+// The tile's pencils from the LDS slots, one pencil per wave (fluidx.c:
+// 61-95): face B to cell centre, tvd1, the new u back into su.
+template <typename T, int K>
+__device__ __forceinline__ void fluidx_lds_pencils(T *su, const T *sb, const T *sb2n, const T *sbk, int n, int ns,
+                                                   const T *__restrict__ dtp)
+{
+    const int L = n / K;
+    const int w = threadIdx.x >> 6;
+    if (w < ns) {
+        const int l = lane_id();
+        const int lc = l % L;  // lanes >= L shadow lane l % L and store nothing
+        const int src_next = lc + 1 == L ? 0 : lc + 1, src_prev = lc == 0 ? L - 1 : lc - 1;
+        const T dt = *dtp;
+        // the lane's K cells as 16-byte LDS reads (lane stride K values:
+        // scalar reads of that stride collide in the banks)
+        T uu[K][5], bxc[K][3];
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            T x[K];
+            load_cells<T, K>(su + (w * 5 + m) * n + lc * K, x);
+#pragma unroll
+            for (int q = 0; q < K; q++) uu[q][m] = x[q];
+        }
+        {
+            T b1[K], b1p[K], b2[K], b2p[K], b3[K], b3p[K];
+            load_cells<T, K>(sb + (w * 3 + 0) * n + lc * K, b1);
+            load_cells<T, K>(sb + (w * 3 + 1) * n + lc * K, b2);
+            load_cells<T, K>((w + 1 < ns ? sb + ((w + 1) * 3 + 1) * n : sb2n) + lc * K, b2p);
+            load_cells<T, K>(sb + (w * 3 + 2) * n + lc * K, b3);
+            load_cells<T, K>(sbk + w * n + lc * K, b3p);
+            shift_next<T, K>(b1, b1p, src_next);  // b1 at i+1, periodic along the pencil
+#pragma unroll
+            for (int q = 0; q < K; q++) {
+                // face B to cell centre (fluidx.c:71-81)
+                bxc[q][0] = (T)0.5 * b1[q] + (T)0.5 * b1p[q];
+                bxc[q][1] = (T)0.5 * b2[q] + b2p[q] / (T)2;
+                bxc[q][2] = (T)0.5 * b3[q] + b3p[q] / (T)2;
+            }
+        }
+        tvd1_pencil<T, K>(uu, bxc, dt, src_next, src_prev);
+        if (l < L) {
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                T x[K];
+#pragma unroll
+                for (int q = 0; q < K; q++) x[q] = uu[q][m];
+                store_cells<T, K>(su + (w * 5 + m) * n + lc * K, x);
+            }
+        }
+    }
+}
+
+// The new u of a y/z tile back to HBM, component by component (the rows of
+// one component leave together, which lets L2 merge the row stores of
+// neighbouring tiles): a thread gathers the VE slots of one vector of row i,
+// as in the loader.
+template <typename T, int FR>
+__device__ __forceinline__ void fluidx_yz_store(T *__restrict__ u, const T *su, const SoaBox &bx, const TileCtx &t,
+                                                int n)
+{
+    constexpr int VE = Vec16<T>::n;
+    const long N = bx.N;
+    const int H = t.ns / VE, tid = threadIdx.x, nt = blockDim.x;
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+        T *uc = u + cu<FR>(c) * N + t.j0 + t.kb;
+        for (int e = tid; e < n * H; e += nt) {
+            const int i = H == 2 ? e >> 1 : e, h = e - i * H;
+            const T *sv = su + (h * VE * 5 + c) * n + i;
+            V16<T> v;
+            if constexpr (sizeof(T) == 4)
+                v = make_float4(sv[0], sv[5 * n], sv[10 * n], sv[15 * n]);
+            else
+                v = make_double2(sv[0], sv[5 * n]);
+            *reinterpret_cast<V16<T> *>(uc + (long)i * bx.f.si + h * VE) = v;
+        }
+    }
+}
+
 // CFLM (x frame only): 0 = plain sweep; 1 / 2 = also the CFL speed of the
 // new state (calcfl.c:45-56, face B / cell-averaged B of OpenMP/mhd.f90:
 // 112-120), the block's max written to cfl_part[tile] - the sweep is the
@@ -270,26 +390,10 @@ __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__re
         const int H = ns / VE;
         for (int e = tid; e < n * H; e += nt) {
             const int i = H == 2 ? e >> 1 : e, h = e - i * H;
-            const long ro = (long)i * f.si + t.j0 + h * VE;
             const bool last = h == H - 1;
-            V16<T> vu[5], vb[4];
-#pragma unroll
-            for (int c = 0; c < 5; c++) vu[c] = *reinterpret_cast<const V16<T> *>(u + cu<FR>(c) * N + t.kb + ro);
-#pragma unroll
-            for (int c = 0; c < 3; c++) vb[c] = *reinterpret_cast<const V16<T> *>(b + cb<FR>(c) * N + t.kb + ro);
-            vb[3] = *reinterpret_cast<const V16<T> *>(b + Comp<FR>::b[2] * N + t.kpb + ro);
-            T b2n = (T)0;
-            if (last) b2n = b[Comp<FR>::b[1] * N + t.kb + (long)i * f.si + jn];
-#pragma unroll
-            for (int v = 0; v < VE; v++) {
-                const int s = h * VE + v;
-#pragma unroll
-                for (int c = 0; c < 5; c++) su[(s * 5 + c) * n + i] = vget<T>(vu[c], v);
-#pragma unroll
-                for (int c = 0; c < 3; c++) sb[(s * 3 + c) * n + i] = vget<T>(vb[c], v);
-                sbk[s * n + i] = vget<T>(vb[3], v);
-            }
-            if (last) sb2n[i] = b2n;
+            YzRow<T> r;
+            yz_row_load<T, FR>(r, u, b, bx, t, jn, i, h, last);
+            yz_row_scatter<T>(r, su, sb, sb2n, sbk, n, i, h, last);
         }
     } else {
         // general tile (partial, or vectors not aligned): one value per item
@@ -322,50 +426,7 @@ __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__re
         }
     }
     __syncthreads();
-
-    const int w = threadIdx.x >> 6;
-    if (w < ns) {
-        const int l = lane_id();
-        const int lc = l % L;  // lanes >= L shadow lane l % L and store nothing
-        const int src_next = lc + 1 == L ? 0 : lc + 1, src_prev = lc == 0 ? L - 1 : lc - 1;
-        const T dt = *dtp;
-        // the lane's K cells as 16-byte LDS reads (lane stride K values:
-        // scalar reads of that stride collide in the banks)
-        T uu[K][5], bxc[K][3];
-#pragma unroll
-        for (int m = 0; m < 5; m++) {
-            T x[K];
-            load_cells<T, K>(su + (w * 5 + m) * n + lc * K, x);
-#pragma unroll
-            for (int q = 0; q < K; q++) uu[q][m] = x[q];
-        }
-        {
-            T b1[K], b1p[K], b2[K], b2p[K], b3[K], b3p[K];
-            load_cells<T, K>(sb + (w * 3 + 0) * n + lc * K, b1);
-            load_cells<T, K>(sb + (w * 3 + 1) * n + lc * K, b2);
-            load_cells<T, K>((w + 1 < ns ? sb + ((w + 1) * 3 + 1) * n : sb2n) + lc * K, b2p);
-            load_cells<T, K>(sb + (w * 3 + 2) * n + lc * K, b3);
-            load_cells<T, K>(sbk + w * n + lc * K, b3p);
-            shift_next<T, K>(b1, b1p, src_next);  // b1 at i+1, periodic along the pencil
-#pragma unroll
-            for (int q = 0; q < K; q++) {
-                // face B to cell centre (fluidx.c:71-81)
-                bxc[q][0] = (T)0.5 * b1[q] + (T)0.5 * b1p[q];
-                bxc[q][1] = (T)0.5 * b2[q] + b2p[q] / (T)2;
-                bxc[q][2] = (T)0.5 * b3[q] + b3p[q] / (T)2;
-            }
-        }
-        tvd1_pencil<T, K>(uu, bxc, dt, src_next, src_prev);
-        if (l < L) {
-#pragma unroll
-            for (int m = 0; m < 5; m++) {
-                T x[K];
-#pragma unroll
-                for (int q = 0; q < K; q++) x[q] = uu[q][m];
-                store_cells<T, K>(su + (w * 5 + m) * n + lc * K, x);
-            }
-        }
-    }
+    fluidx_lds_pencils<T, K>(su, sb, sb2n, sbk, n, ns, dtp);
     __syncthreads();
     if constexpr (CFLM != 0) {
         // CFL speed of the new state: after the barrier (a scheduling
@@ -416,24 +477,7 @@ __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__re
                 *reinterpret_cast<const V16<T> *>(su + (s * 5 + c) * n + i4 * VE);
         }
     } else if (FR != 0 && t.vec) {
-        // component by component (the rows of one component leave together,
-        // which lets L2 merge the row stores of neighbouring tiles): a thread
-        // gathers the VE slots of one vector of row i, as in the loader
-        const int H = ns / VE;
-#pragma unroll
-        for (int c = 0; c < 5; c++) {
-            T *uc = u + cu<FR>(c) * N + t.j0 + t.kb;
-            for (int e = tid; e < n * H; e += nt) {
-                const int i = H == 2 ? e >> 1 : e, h = e - i * H;
-                const T *sv = su + (h * VE * 5 + c) * n + i;
-                V16<T> v;
-                if constexpr (sizeof(T) == 4)
-                    v = make_float4(sv[0], sv[5 * n], sv[10 * n], sv[15 * n]);
-                else
-                    v = make_double2(sv[0], sv[5 * n]);
-                *reinterpret_cast<V16<T> *>(uc + (long)i * f.si + h * VE) = v;
-            }
-        }
+        fluidx_yz_store<T, FR>(u, su, bx, t, n);
     } else {
         for (int e = tid; e < ns * 5 * n; e += nt) {
             const int r = fdiv(e, bx.n_div), i = e - r * n, s = r / 5, c = r - 5 * s;
@@ -977,6 +1021,14 @@ __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__rest
 // ---------------------------------------------------------------------------
 template <int W> __host__ __device__ constexpr int advect_x2_words(int n) { return (5 * W + 2) * n; }
 
+// log2 of the plane pairs per group in advect_x2_tile's tile order.  At
+// 256^3 a group of 16 plane pairs is one XCD's share of the launch (1024
+// tiles); measured on MI355X: 0.130-0.133 ms per sweep with 4 plane pairs,
+// 0.127-0.129 with 8, 0.122-0.125 with 16, 0.121-0.125 with 32
+#ifndef MHD_X2_GROUP_LOG
+#define MHD_X2_GROUP_LOG 4
+#endif
+
 template <typename T, int K, int W, bool VEC>
 __device__ __forceinline__ void advect_x2_tile(const T *__restrict__ u, const T *__restrict__ bin,
                                                T *__restrict__ bout, const SoaBox &bx, const T *__restrict__ dtp,
@@ -985,20 +1037,21 @@ __device__ __forceinline__ void advect_x2_tile(const T *__restrict__ u, const T 
     constexpr int FR = 0;
     const Frame &f = bx.f;
     const int n = f.ni, L = n / K;
-    // tiles in groups of 8 plane pairs by all pencil tiles, the plane pairs
-    // fastest: a tile's neighbours along j (pencil j0-1, j0+W) and along k
-    // (planes k-1, k+2) are mostly blocks close in launch order, so their
-    // rows come from L2
-    const int nk2 = (f.nk + 1) / 2, nfull = nk2 >> 3, ing = ntj << 3;
+    // tiles in groups of 2^G plane pairs by all pencil tiles, the plane
+    // pairs fastest: a tile's neighbours along j (pencil j0-1, j0+W) and
+    // along k (planes k-1, k+2) are mostly blocks close in launch order, so
+    // their rows come from L2
+    constexpr int G = MHD_X2_GROUP_LOG;
+    const int nk2 = (f.nk + 1) / 2, nfull = nk2 >> G, ing = ntj << G;
     int jt, kq;
     if (tile < nfull * ing) {
         const int kg = tile / ing, r = tile - kg * ing;
-        jt = r >> 3;
-        kq = (kg << 3) + (r & 7);
+        jt = r >> G;
+        kq = (kg << G) + (r & ((1 << G) - 1));
     } else {
-        const int rem = nk2 - (nfull << 3), r = tile - nfull * ing;
+        const int rem = nk2 - (nfull << G), r = tile - nfull * ing;
         jt = r / rem;
-        kq = (nfull << 3) + (r - jt * rem);
+        kq = (nfull << G) + (r - jt * rem);
     }
     const int j0 = jt * W, ns = min(W, f.nj - j0);
     const int k = 2 * kq, P = k + 1 < f.nk ? 2 : 1;

@@ -147,7 +147,13 @@ def test_config1_alfven_32cube_200_steps(M, oracle):
                                                      # pencils of 2 and 3 cells; a partial 64-pencil tile
                                                      ((70, 2, 3), 505, 0.4, 3, math.inf),
                                                      # an odd plane count
-                                                     ((64, 100, 45), 506, 0.35, 2, math.inf)])
+                                                     ((64, 100, 45), 506, 0.35, 2, math.inf),
+                                                     # y/z fluid sweeps of pencils that fill the wave
+                                                     # (64K cells) over pairs of tiles: odd tile counts
+                                                     # (the last workgroup has one), K = 1, 4, 2
+                                                     ((24, 64, 3), 507, 0.4, 3, math.inf),
+                                                     ((8, 256, 5), 508, 0.4, 2, math.inf),
+                                                     ((56, 128, 64), 509, 0.3, 2, math.inf)])
 def test_ctx_steps_match_oracle(M, oracle, shape, seed, amp, steps, tf):
     nx, ny, nz = shape
     u0, b0 = random_state(nx, ny, nz, seed=seed, amp=amp)
