@@ -89,6 +89,22 @@ def cpu_baseline(u0, b0, n, seconds_hint=30.0):
             "sample": f"1 step of the {n}^3 workload (same initial state), {dt:.1f} s wall"}
 
 
+def cpu_baseline_omp(u0, b0, n):
+    """The oracle restatement (the reference's C evaluation order) with its
+    pencil loops spread over OpenMP threads, as the reference's Fortran OpenMP
+    driver spreads them (OpenMP/mhd.f90), on one step of the same workload:
+    the multi-core CPU figure beside the serial reference one."""
+    import oracle_lib
+    orc = oracle_lib.Oracle()
+    u, b = u0.copy(), b0.copy()
+    t0 = time.perf_counter()
+    orc.run(u, b, n, n, n, 1)
+    dt = time.perf_counter() - t0
+    return {"value": n ** 3 / dt, "unit": "zone-updates/s", "cores": orc.threads(), "kind": "port",
+            "sample": f"1 step of the {n}^3 workload (same initial state), {dt:.2f} s wall, "
+                      f"OpenMP over pencils"}
+
+
 def bench_batched(mhdgpu, steps=100, nbox=64, seed=3):
     """The batched workload: nbox independent boxes, N uniform over
     {32,48,64,96,128} (seeded), each the linear Alfven wave with its own seeded
@@ -369,6 +385,7 @@ def main():
     if not args.no_cpu:
         log("bench: CPU baseline")
         out["cpu_baseline"] = cpu_baseline(u0, b0, n)
+        out["cpu_baseline_omp"] = cpu_baseline_omp(u0, b0, n)
     print(json.dumps(out), flush=True)
 
 

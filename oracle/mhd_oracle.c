@@ -17,6 +17,7 @@
  *     rounding.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -31,6 +32,10 @@
 #include "mhd_oracle_impl.h"
 #undef R
 #undef SFX
+
+/* Threads of the oracle's OpenMP pencil loops (OMP_NUM_THREADS or the
+ * runtime's default), for reporting the CPU timing beside the GPU one. */
+int mhd_oracle_threads(void) { return omp_get_max_threads(); }
 
 /* Initial-state reader of Heterogeneous_MHD/C/init.c:33-54: the file holds
  * 8 floats per cell (u1..u5, b1..b3); the reader's loop runs i outermost and

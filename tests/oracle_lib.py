@@ -43,6 +43,7 @@ class Oracle:
             _bind(lib, "mhd_oracle_run" + sfx, None, [p, p, _I, _I, _I, _I, R, _I, ctypes.c_void_p])
         _bind(lib, "mhd_oracle_read_ref_order", None, [_f32p, _f32p, _f32p, _I, _I, _I])
         _bind(lib, "mhd_oracle_monitors_f32", None, [_f32p, _f32p, _I, _I, _I, _f64p])
+        _bind(lib, "mhd_oracle_threads", _I, [])
 
     @staticmethod
     def _sfx(a):
@@ -66,6 +67,10 @@ class Oracle:
 
     def calcfl(self, u, b, nx, ny, nz, avg_b=False):
         return getattr(self.lib, "mhd_oracle_calcfl" + self._sfx(u))(u, b, nx, ny, nz, int(avg_b))
+
+    def threads(self):
+        """Threads of the OpenMP pencil loops."""
+        return self.lib.mhd_oracle_threads()
 
     def run(self, u, b, nx, ny, nz, nsteps, tf=float("inf"), avg_b=False):
         """nsteps reference loop iterations in place; returns the dt sequence."""
