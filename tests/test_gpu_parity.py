@@ -76,6 +76,18 @@ def test_advectbyzx_matches_oracle(M, oracle, shape):
     assert same(bg, bo)
 
 
+@pytest.mark.parametrize("shape", [(64, 40, 9), (256, 3, 70)])
+def test_advectbyzx_at_rest_leaves_the_field(M, shape):
+    # zero velocity: every tvdb flux is exactly zero, so b must come back bit
+    # for bit (the x-frame tiles and the y/z column streams alike)
+    nx, ny, nz = shape
+    print("advectbyzx at rest", shape, flush=True)
+    u0, b0 = random_state(nx, ny, nz, seed=300 + nx, amp=0.5)
+    u0.reshape(-1, 5)[:, 1:4] = 0.0
+    ug, bg = gpu_op(M, "advectbyzx", u0, b0, nx, ny, nz, F32(0.37))
+    assert bitwise(ug, u0) and bitwise(bg, b0)
+
+
 def test_single_ops_on_reference_fixtures(M, oracle, golden, fixture16, fixture32):
     u0, b0 = fixture16
     ug, _ = gpu_op(M, "fluidx", u0, b0, 16, 16, 16, F32(0.3))

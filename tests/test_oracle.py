@@ -7,7 +7,8 @@
   shapes;
 * mathematics: transposes are involutive permutations, uniform states are
   fixed points, fluidx conserves mass/momentum/energy, advectbyzx keeps the
-  discrete div B and leaves u alone, the CFL speed has its closed form, the
+  discrete div B and leaves u alone, a field at rest is left unchanged by
+  advectbyzx (zero tvdb fluxes), the CFL speed has its closed form, the
   linear Alfven wave travels at v_A = 1 with second-order convergence, and the
   FP64 instantiation agrees with FP32 to rounding.
 """
@@ -162,6 +163,24 @@ def test_fluidx_conserves_and_advect_keeps_divb(oracle):
     oracle.advectbyzx(u, b, nx, ny, nz, F32(0.2))
     assert bitwise(u, u2)
     assert np.max(np.abs(div(b) - d0)) < 2e-6
+
+
+def test_advect_at_rest_leaves_the_field(oracle):
+    # tvdb at vg = 0 (tvdb.c:10-91): the upwind product, the half-step field
+    # and the limited flux are all exactly zero, so every face field and the
+    # EMF correction of b1 come out unchanged bit for bit, whatever b is
+    nx, ny, nz = 12, 10, 9
+    u, b = random_state(nx, ny, nz, seed=61, amp=0.5)
+    U = u.reshape(-1, 5)
+    U[:, 1:4] = 0.0
+    u0, b0 = u.copy(), b.copy()
+    oracle.advectbyzx(u, b, nx, ny, nz, F32(0.37))
+    assert bitwise(u, u0) and bitwise(b, b0)
+    # the check can fail: a small v_x (the sweep's own velocity, u2/u1)
+    # moves the field
+    U[:, 1] = F32(1e-3)
+    oracle.advectbyzx(u, b, nx, ny, nz, F32(0.37))
+    assert not bitwise(b, b0)
 
 
 def test_cfl_closed_form(oracle):
