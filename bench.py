@@ -281,7 +281,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it", file=sys.stderr)
+        print("bench.py: the library runs on one GPU only; runs on several GPUs are not supported", file=sys.stderr)
         sys.exit(2)
 
     import mhdgpu

@@ -2,7 +2,7 @@
  * The oracle: a CPU restatement of the reference's relaxing-TVD MHD timestep
  * (Heterogeneous_MHD/C), built as liboracle_mhd.so.  TEST INFRASTRUCTURE
  * ONLY: tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
- * it as the checker; the product library never links or calls it.
+ * it to check results against; the product library never links or calls it.
  *
  * Parity pins (each function is checked by tests/test_oracle.py):
  *   - against the reference's own C sources compiled into oracle/_ref
