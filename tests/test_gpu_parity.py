@@ -252,6 +252,36 @@ def test_ctx_fp64_matches_fp64_oracle(M, oracle):
     sim.close()
 
 
+_FORM_ORACLE64 = {}
+
+
+# The FP64 state through the vector paths and every y/z advectbyzx form: at
+# (128, 64, 10) the x pencils of 128 cells take K = 2 with 16-byte loads and
+# the y pencils of 64 cells the full-wave two-vector tiles; at (130, 40, 33)
+# the fused stream's partial third wave and odd plane counts
+@pytest.mark.parametrize("form", ["tiles", "two_pass", "fused"])
+@pytest.mark.parametrize("shape", [(128, 64, 10), (130, 40, 33)])
+def test_fp64_forms_match_fp64_oracle(M, oracle, shape, form):
+    nx, ny, nz = shape
+    print("fp64 advect form", form, shape, flush=True)
+    u0, b0 = random_state(nx, ny, nz, seed=610 + nx, amp=0.35)
+    u0, b0 = u0.astype(np.float64), b0.astype(np.float64)
+    if shape not in _FORM_ORACLE64:
+        uo, bo = u0.copy(), b0.copy()
+        do = oracle.run(uo, bo, nx, ny, nz, 2)
+        _FORM_ORACLE64[shape] = (uo, bo, do)
+    uo, bo, do = _FORM_ORACLE64[shape]
+    sim = M.Simulation(nx, ny, nz, precision=8)
+    sim.set_advect_form(form)
+    sim.set_state(u0, b0)
+    sim.step(2)
+    ug, bg = sim.get_state()
+    dg = sim.dts()
+    sim.close()
+    assert bitwise(dg, do)
+    assert same(ug, uo) and same(bg, bo)
+
+
 def test_avg_b_cfl_variant_in_driver(M, oracle):
     n = 16
     u0, b0 = random_state(n, n, n, seed=602, amp=0.4)
