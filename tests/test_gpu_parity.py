@@ -350,3 +350,28 @@ def test_full_size_256_step_properties_and_parity(M, oracle):
     assert same(ug, uo) and same(bg, bo)
     assert m0[5] < 1e-5 and m1[5] < 1e-5  # div B from curl A stays at rounding level
     assert abs(m1[0] - m0[0]) <= 1e-6 * m0[0]
+
+
+@pytest.mark.timeout(900)
+def test_max_size_512_step_matches_oracle(M, oracle):
+    """The largest configuration's size: one 512^3 step (pencils of 512 cells,
+    K = 8; 5 x 512^3 = 6.7e8 values, int32 indices) against the oracle, bit
+    for bit, with mass conserved."""
+    n = 512
+    print("512^3: state", flush=True)
+    u0, b0 = random_state(n, n, n, seed=512, amp=0.2)
+    sim = M.Simulation(n, n, n)
+    sim.set_state(u0, b0)
+    m0 = sim.monitors()
+    sim.step(1)
+    ug, bg = sim.get_state()
+    m1 = sim.monitors()
+    dg = sim.dts()
+    sim.close()
+    print("512^3: oracle", flush=True)
+    uo, bo = u0.copy(), b0.copy()
+    del u0, b0
+    do = oracle.run(uo, bo, n, n, n, 1)
+    assert bitwise(dg, do)
+    assert same(ug, uo) and same(bg, bo)
+    assert abs(m1[0] - m0[0]) <= 1e-6 * m0[0]
