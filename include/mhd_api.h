@@ -74,6 +74,15 @@ int mhd_fluidx(float *u, const float *b, const int *nx, const int *ny, const int
  * correction of b1; the reference's N^3 scratch `temp` is not needed */
 int mhd_advectbyzx(const float *u, float *b, const int *nx, const int *ny, const int *nz, const float *dt);
 
+/* mhd_advectbyzx without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The input
+ * copy of b lives in a fixed per-device array of 2^21 cells (128^3; 24 MiB,
+ * held until the process ends); larger boxes return MHD_ERR_SHAPE.  Every
+ * call on one GPU shares that array, so calls that overlap on one GPU (from
+ * several host threads) race; results equal mhd_advectbyzx's. */
+int mhd_advectbyzx_noalloc(const float *u, float *b, const int *nx, const int *ny, const int *nz, const float *dt);
+
 /* replaces cuda_transpose12 (HIP/cu/cuda_transpose12.cu:12, HIP/cuda_transpose12.cpp:13):
  * (i,j,k)->(j,i,k) with u components (1,3,2,4,5) and b (2,1,3); the result
  * is written to ut,bt and copied back into u,b as in the reference, so after

@@ -101,6 +101,34 @@ static int bcopy_scratch(size_t bytes, void **p)
     return MHD_OK;
 }
 
+// Fixed scratch of mhd_advectbyzx_noalloc: the input copy of b that the
+// sweep reads while it writes the caller's b, for boxes of at most
+// kNoAllocCells cells (3 floats each: 24 MiB per device, held until the
+// process ends).  In host code `scratch` names a host-side stand-in, so the
+// device address of each GPU's copy is looked up once per device and kept.
+static constexpr long kNoAllocCells = 1L << 21;  // 128^3
+static __device__ __align__(256) float scratch[3 * kNoAllocCells];
+static std::mutex g_noalloc_mu;
+static float *g_noalloc_ptr[64];
+
+static int noalloc_scratch(float **p)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= 64) return fail(MHD_ERR_ARG, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void *q = nullptr;
+        e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        if (!q) return fail(MHD_ERR_HIP, "hipGetSymbolAddress(scratch) gave a null address");
+        g_noalloc_ptr[dev] = (float *)q;
+    }
+    *p = g_noalloc_ptr[dev];
+    return MHD_OK;
+}
+
 static int sync_check(hipError_t e, const char *where)
 {
     if (e != hipSuccess) return hip_fail(e, where);
@@ -209,6 +237,25 @@ int mhd_advectbyzx(const float *u, float *b, const int *nx, const int *ny, const
     hipError_t e = hipMemcpyAsync(bin, b, bytes, hipMemcpyDeviceToDevice, 0);
     if (e == hipSuccess) e = launch_advect<float>(u, (const float *)bin, b, 0, *nx, *ny, *nz, dt, 0);
     return sync_check(e, "mhd_advectbyzx");
+}
+
+int mhd_advectbyzx_noalloc(const float *u, float *b, const int *nx, const int *ny, const int *nz, const float *dt)
+{
+    if (!u || !b || !nx || !ny || !nz || !dt) return fail(MHD_ERR_ARG, "mhd_advectbyzx_noalloc: null argument");
+    int rc = check_shape(*nx, *ny, *nz);
+    if (rc) return rc;
+    const long cells = (long)*nx * *ny * *nz;
+    if (cells > kNoAllocCells)
+        return fail(MHD_ERR_SHAPE,
+                    "mhd_advectbyzx_noalloc: %dx%dx%d = %ld cells exceed its fixed scratch of %ld cells; "
+                    "use mhd_advectbyzx",
+                    *nx, *ny, *nz, cells, kNoAllocCells);
+    float *bin = nullptr;
+    rc = noalloc_scratch(&bin);
+    if (rc) return rc;
+    hipError_t e = launch_copy_f32(bin, b, 3 * cells, 0);
+    if (e == hipSuccess) e = launch_advect<float>(u, bin, b, 0, *nx, *ny, *nz, dt, 0);
+    return sync_check(e, "mhd_advectbyzx_noalloc");
 }
 
 static int transpose_common(int which, float *ut, float *bt, float *u, float *b, const int *nx, const int *ny,

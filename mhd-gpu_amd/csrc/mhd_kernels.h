@@ -36,6 +36,8 @@ template <typename T> hipError_t prepare_sweeps(int nx, int ny, int nz);
 hipError_t launch_zero_u64(unsigned long long *p, int n, hipStream_t st);
 // plain device copy kernel (bytes a multiple of 16)
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st);
+// n floats from src to dst by a kernel (no runtime copy engine)
+hipError_t launch_copy_f32(float *dst, const float *src, long n, hipStream_t st);
 // *maxbits (zeroed by the caller) = FP64 bits of max |1 - D*rcp_f64(D)| over all float-valued D
 hipError_t launch_rcp_scan(unsigned long long *maxbits, hipStream_t st);
 

@@ -470,6 +470,20 @@ hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st)
     return hipGetLastError();
 }
 
+// n floats, any count and 4-byte alignment
+__global__ void __launch_bounds__(256) copy_f32_kernel(float *__restrict__ dst, const float *__restrict__ src, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t launch_copy_f32(float *dst, const float *src, long n, hipStream_t st)
+{
+    const long nb = (n + 255) / 256;
+    hipLaunchKernelGGL(copy_f32_kernel, dim3(nb < 4096 ? (nb > 0 ? nb : 1) : 4096), dim3(256), 0, st, dst, src, n);
+    return hipGetLastError();
+}
+
 // max |1 - D*rcp(D)| of v_rcp_f64 over every finite nonzero float-valued
 // D, as FP64 bits (non-negative doubles order like their bit patterns); the
 // bound the FP32 limiter quotient (limiter_quotient) rests on

@@ -46,6 +46,7 @@ SIGNATURES = {
     "mhd_selfcheck": (_I, [ctypes.POINTER(ctypes.c_double)]),
     "mhd_fluidx": (_I, [_P, _P, _IP, _IP, _IP, _P]),
     "mhd_advectbyzx": (_I, [_P, _P, _IP, _IP, _IP, _P]),
+    "mhd_advectbyzx_noalloc": (_I, [_P, _P, _IP, _IP, _IP, _P]),
     "mhd_transpose12": (_I, [_P, _P, _P, _P, _IP, _IP, _IP]),
     "mhd_transpose13": (_I, [_P, _P, _P, _P, _IP, _IP, _IP]),
     "mhd_cfl": (_I, [_P, _P, _IP, _IP, _IP, _I, _P, _FP]),
@@ -173,6 +174,11 @@ def fluidx(u, b, nx, ny, nz, dt):
 def advectbyzx(u, b, nx, ny, nz, dt):
     """CT advection of b2, b3 along i (Heterogeneous_MHD/C/advectbyzx.c:14)."""
     check(lib().mhd_advectbyzx(u.ptr, b.ptr, _iref(nx), _iref(ny), _iref(nz), dt.ptr))
+
+
+def advectbyzx_noalloc(u, b, nx, ny, nz, dt):
+    """advectbyzx without device allocation (fixed per-device scratch, at most 128^3 cells)."""
+    check(lib().mhd_advectbyzx_noalloc(u.ptr, b.ptr, _iref(nx), _iref(ny), _iref(nz), dt.ptr))
 
 
 def transpose12(ut, bt, u, b, nx, ny, nz):
