@@ -291,7 +291,7 @@ int mhd_cfl(const float *u, const float *b, const int *nx, const int *ny, const 
     int rc = step_scratch(&scr);
     if (rc) return rc;
     hipError_t e = launch_cfl<float>(u, b, *nx, *ny, *nz, avg_b, scr, 0);
-    if (e == hipSuccess) e = launch_cfl_value<float>(scr, d_cfl, 0);
+    if (e == hipSuccess) e = launch_cfl_value<float>(scr, d_cfl, u, b, *nx, *ny, *nz, avg_b ? 1 : 0, 0);
     rc = sync_check(e, "mhd_cfl");
     if (rc) return rc;
     if (h_cfl) return mhd_memcpy_d2h(h_cfl, d_cfl, sizeof(float));

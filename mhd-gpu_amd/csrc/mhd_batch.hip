@@ -332,23 +332,47 @@ __global__ void __launch_bounds__(256) batch_cfl_kernel(const BoxDesc *__restric
     const CflItem it = items[blockIdx.x];
     const BoxDesc &bd = boxes[it.box];
     float c = cfl_range<float, AVG>(bd.u, bd.b, bd.g, (long)it.e0 + threadIdx.x, (long)it.e0 + it.ne, blockDim.x);
-    c = wave_max(c);
+    c = wave_cfl_max(c);
     __shared__ float red[4];
     if (lane_id() == 0) red[threadIdx.x >> 6] = c;
     __syncthreads();
     if (threadIdx.x == 0) {
         float m = red[0];
-        for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = fmaxf(m, red[i]);
+        for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = cfl_max(m, red[i]);
         atomicMax(cmax + it.box, (unsigned long long)__float_as_uint(m));
     }
 }
 
 // per-box dt (Heterogeneous_MHD/C/mhd.c:63-65) for the active slots; clears
-// the CFL maxima and the sweep queues for this step
-__global__ void batch_dt_kernel(const int *__restrict__ active, int nactive, unsigned long long *cmax, float *dt,
-                                float *t, double *t64, const float *tf, float *dts, int cap, int *iter, unsigned *ctr,
-                                int nctr)
+// the CFL maxima and the sweep queues for this step.  A box whose CFL max is
+// NaN (NaN-sticky, cfl_max) gets the reference's sequential max over its
+// state instead (cfl_after_last_nan), by the whole block.
+__global__ void __launch_bounds__(256) batch_dt_kernel(const BoxDesc *__restrict__ boxes, int avg,
+                                                       const int *__restrict__ active, int nactive,
+                                                       unsigned long long *cmax, float *dt, float *t, double *t64,
+                                                       const float *tf, float *dts, int cap, int *iter, unsigned *ctr,
+                                                       int nctr)
 {
+    __shared__ int nan_box[256];
+    __shared__ int n_nan;
+    for (int base = 0; base < nactive; base += blockDim.x) {
+        const int a = base + (int)threadIdx.x;
+        const float ca = a < nactive ? __uint_as_float((unsigned)cmax[active[a]]) : 0.0f;
+        const bool bad = ca != ca;
+        if (!__syncthreads_or(bad)) continue;
+        if (threadIdx.x == 0) n_nan = 0;
+        __syncthreads();
+        if (bad) nan_box[atomicAdd(&n_nan, 1)] = active[a];
+        __syncthreads();
+        for (int q = 0; q < n_nan; q++) {
+            const BoxDesc &bd = boxes[nan_box[q]];
+            const long ncell = (long)bd.g.nx * bd.g.ny * bd.g.nz;
+            const float c = avg ? cfl_after_last_nan<float>(ncell, SoaSpeed<float, true>{bd.u, bd.b, bd.g})
+                                : cfl_after_last_nan<float>(ncell, SoaSpeed<float, false>{bd.u, bd.b, bd.g});
+            if (threadIdx.x == 0) cmax[nan_box[q]] = (unsigned long long)__float_as_uint(c);
+        }
+        __syncthreads();
+    }
     for (int a = threadIdx.x; a < nactive; a += blockDim.x) {
         const int bx = active[a];
         const float c = __uint_as_float((unsigned)cmax[bx]);
@@ -1154,7 +1178,7 @@ int Engine::enqueue_step()
             // accumulator: the previous step's last sweep put it there (or the
             // standalone CFL run by step() after the state was set)
             if (e == hipSuccess) {
-                hipLaunchKernelGGL(batch_dt_kernel, dim3(1), dim3(256), 0, S, d_active_ + G.act_off, G.act_n, d_cmax_,
+                hipLaunchKernelGGL(batch_dt_kernel, dim3(1), dim3(256), 0, S, d_boxes_, avg_, d_active_ + G.act_off, G.act_n, d_cmax_,
                                    d_dt_, d_t_, d_t64_, d_tf_, d_dts_, cap_, d_iter_, d_ctr_, 0);
                 e = hipGetLastError();
             }
@@ -1226,7 +1250,7 @@ int Engine::enqueue_step()
         hipLaunchKernelGGL((batch_cfl_kernel<false>), dim3(ncfl_), dim3(256), 0, st_, d_boxes_, d_cfl_, d_cmax_);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(batch_dt_kernel, dim3(1), dim3(256), 0, st_, d_active_, (int)active_.size(), d_cmax_, d_dt_,
+        hipLaunchKernelGGL(batch_dt_kernel, dim3(1), dim3(256), 0, st_, d_boxes_, avg_, d_active_, (int)active_.size(), d_cmax_, d_dt_,
                            d_t_, d_t64_, d_tf_, d_dts_, cap_, d_iter_, d_ctr_, 12 * kQueues);
         e = hipGetLastError();
     }

@@ -7,11 +7,12 @@
 // every operation is FP64 (the FP64 instantiation used as an error yardstick).
 //
 // The one liberty taken: the reference's helper a*X + b*Y + c
-// (Heterogeneous_MHD/C/includefunc_spu.h:73-94) is called with c = 0 and,
-// for the c*w products, b = 0; the "+ 0" and "0*w" terms only change the sign
-// of an exact zero result (and turn an infinity into NaN), so they are left
-// out.  Results therefore equal the reference's numerically (-0 == +0); every
-// non-zero value matches bit for bit.
+// (Heterogeneous_MHD/C/includefunc_spu.h:73-94) is always called with
+// c = 0; that "+ 0" only turns an exact -0 into +0, so it is left out.
+// Results therefore equal the reference's numerically (-0 == +0, NaN where
+// the reference has NaN); every non-zero value matches bit for bit.  (The
+// "0*w" terms of its calls with b = 0 are kept: they make an infinite w
+// give NaN.)
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -27,6 +28,14 @@ namespace mhd {
 struct Frame {
     int ni, nj, nk;     // pencil length, pencils along j, pencils along k
     long si, sj, sk;    // strides of i, j, k in cells
+};
+
+// SoA layout of one box: cell (i,j,k) at i + j*sy + k*sz of each plane,
+// planes N apart; sy >= nx and sz >= sy*ny leave padding that breaks
+// power-of-two strides
+struct SoaGrid {
+    int nx, ny, nz;
+    long sy, sz, N;
 };
 
 // Frame component maps: frame component c lives in physical component map[c].
@@ -239,12 +248,23 @@ template <typename T>
 __device__ __forceinline__ T cfl_speed(T u1, T u2, T u3, T u4, T u5, T B)
 {
     const T g = gamma_c<T>();
-    // calcfl.c:45-49 takes the largest of |u2/u1|, |u3/u1|, |u4/u1|.  A
-    // correctly rounded division is monotonic in the dividend, so that is the
-    // quotient of the largest |u_k| (picked by the same comparisons, which
-    // keeps the choice among NaNs too): one division instead of three.
+    // calcfl.c:45-49 takes the largest of |u2/u1|, |u3/u1|, |u4/u1| by the
+    // comparisons of max_3num (:67-91).  For a finite nonzero u1 a correctly
+    // rounded division is monotonic in the dividend, so that is the quotient
+    // of the largest |u_k| picked by the same comparisons (two dividends
+    // that compare unequal may give equal quotients, but then either pick
+    // has that value; a NaN dividend gives a NaN quotient and loses every
+    // comparison either way): one division instead of three.  For u1 = 0 or
+    // infinite the quotients 0/0 and inf/inf are NaN while their dividends
+    // are not, so the three quotients are compared as the reference does.
     const T a1 = fabs(u2), a2 = fabs(u3), a3 = fabs(u4);
-    const T v = (a1 > a2 ? (a1 > a3 ? a1 : a3) : (a2 > a3 ? a2 : a3)) / fabs(u1);
+    T v;
+    if (__builtin_isfinite(u1) && u1 != (T)0) {
+        v = (a1 > a2 ? (a1 > a3 ? a1 : a3) : (a2 > a3 ? a2 : a3)) / fabs(u1);
+    } else {
+        const T q1 = fabs(u2 / u1), q2 = fabs(u3 / u1), q3 = fabs(u4 / u1);
+        v = q1 > q2 ? (q1 > q3 ? q1 : q3) : (q2 > q3 ? q2 : q3);
+    }
     const T S = u2 * u2 + u3 * u3 + u4 * u4;
     const T ps = pstar<T>(u1, u5, S, B);
     const T p = (T)((double)ps - (double)B / 2.0);
@@ -294,6 +314,128 @@ template <typename T> __device__ __forceinline__ T wave_max(T x)
     for (int off = 32; off >= 1; off >>= 1) x = fmax(x, shfl_from(x, lane_id() ^ off));
     return x;
 }
+
+// ---- the CFL max (calcfl.c:55, max_2num in k,j,i order) ----
+// Every CFL speed is +0, positive, +inf or NaN (never -0: v >= +0 and the
+// root is +0 or -0, whose sum with v is v or +0).  The reference's running
+// max_2num(c, t) = c > t ? c : t resets to NaN at a NaN speed and to the
+// next speed after it, so its result depends on where the NaN cells are.
+// The parallel reductions therefore carry a NaN-sticky max - as unsigned
+// bit patterns every NaN lies above +inf and the other speeds keep their
+// order - and a NaN result sends the consumer to the exact sequential form
+// (cfl_after_last_nan).  Without NaN both give the plain maximum.
+template <typename T> __device__ __forceinline__ unsigned long long speed_bits(T x)
+{
+    if constexpr (sizeof(T) == 4)
+        return (unsigned long long)__float_as_uint(x);
+    else
+        return (unsigned long long)__double_as_longlong(x);
+}
+template <typename T> __device__ __forceinline__ T speed_from_bits(unsigned long long x)
+{
+    if constexpr (sizeof(T) == 4)
+        return __uint_as_float((unsigned)x);
+    else
+        return __longlong_as_double((long long)x);
+}
+template <typename T> __device__ __forceinline__ T cfl_max(T a, T b)
+{
+    if constexpr (sizeof(T) == 4)
+        return __uint_as_float(max(__float_as_uint(a), __float_as_uint(b)));
+    else
+        return __longlong_as_double((long long)max((unsigned long long)__double_as_longlong(a),
+                                                   (unsigned long long)__double_as_longlong(b)));
+}
+template <typename T> __device__ __forceinline__ T wave_cfl_max(T x)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) x = cfl_max(x, shfl_from(x, lane_id() ^ off));
+    return x;
+}
+
+// The reference's sequential CFL max over ncell speeds (speed(s) for the
+// cell of linear index s = (k*ny+j)*nx+i) when some speed is NaN: the max
+// of the speeds after the last NaN one, floored at the initial c = 0 (all
+// speeds are >= +0), or NaN when the last cell's speed is NaN.  Called by
+// every thread of the block; two passes over the box (the rare path of a
+// state that already holds NaN or a non-positive density).
+template <typename T, class Speed> __device__ T cfl_after_last_nan(long ncell, const Speed &speed)
+{
+    __shared__ unsigned long long last1, mbits;  // last NaN index + 1 (0: none); max speed bits
+    if (threadIdx.x == 0) {
+        last1 = 0ull;
+        mbits = 0ull;
+    }
+    __syncthreads();
+    long mine = -1;
+    for (long s = threadIdx.x; s < ncell; s += blockDim.x) {
+        const T t = speed(s);
+        if (t != t) mine = s;
+    }
+    if (mine >= 0) atomicMax(&last1, (unsigned long long)(mine + 1));
+    __syncthreads();
+    const long first = (long)last1;  // the cells after the last NaN one
+    T c = (T)0;
+    for (long s = first + threadIdx.x; s < ncell; s += blockDim.x) c = fmax(c, speed(s));
+    atomicMax(&mbits, speed_bits(c));
+    __syncthreads();
+    const T r = first == ncell ? (T)__builtin_nan("") : speed_from_bits<T>(mbits);
+    __syncthreads();  // the shared words are read before any later call resets them
+    return r;
+}
+
+// The CFL speed (calcfl.c:45-54) of the cell of linear index
+// s = (k*ny+j)*nx+i, with the face B at the cell or (AVG) the cell average
+// of calcfl.c:38-43: over a box in the step-level AoS layout, and over a
+// device-resident SoA box
+template <typename T, bool AVG> struct AosSpeed {
+    const T *u, *b;
+    int nx, ny, nz;
+    __device__ T operator()(long s) const
+    {
+        const T *pu = u + 5 * s, *pb = b + 3 * s;
+        T B;
+        if constexpr (AVG) {
+            const int i = (int)(s % nx);
+            const long r = s / nx;
+            const int j = (int)(r % ny), k = (int)(r / ny);
+            const long sip = s - i + (i + 1 == nx ? 0 : i + 1);
+            const long sjp = ((long)k * ny + (j + 1 == ny ? 0 : j + 1)) * nx + i;
+            const long skp = ((long)(k + 1 == nz ? 0 : k + 1) * ny + j) * nx + i;
+            const T bx = (T)((double)(pb[0] + b[3 * sip + 0]) / 2.0);
+            const T by = (T)((double)(pb[1] + b[3 * sjp + 1]) / 2.0);
+            const T bz = (T)((double)(pb[2] + b[3 * skp + 2]) / 2.0);
+            B = bx * bx + by * by + bz * bz;
+        } else {
+            B = pb[0] * pb[0] + pb[1] * pb[1] + pb[2] * pb[2];
+        }
+        return cfl_speed<T>(pu[0], pu[1], pu[2], pu[3], pu[4], B);
+    }
+};
+template <typename T, bool AVG> struct SoaSpeed {
+    const T *u, *b;
+    SoaGrid g;
+    __device__ T operator()(long s) const
+    {
+        const int i = (int)(s % g.nx);
+        const long r = s / g.nx;
+        const int j = (int)(r % g.ny), k = (int)(r / g.ny);
+        const long row = (long)j * g.sy + (long)k * g.sz, c = row + i, N = g.N;
+        T B;
+        if constexpr (AVG) {
+            const long sip = row + (i + 1 == g.nx ? 0 : i + 1);
+            const long sjp = (long)(j + 1 == g.ny ? 0 : j + 1) * g.sy + (long)k * g.sz + i;
+            const long skp = (long)j * g.sy + (long)(k + 1 == g.nz ? 0 : k + 1) * g.sz + i;
+            const T bx = (T)((double)(b[c] + b[sip]) / 2.0);
+            const T by = (T)((double)(b[N + c] + b[N + sjp]) / 2.0);
+            const T bz = (T)((double)(b[2 * N + c] + b[2 * N + skp]) / 2.0);
+            B = bx * bx + by * by + bz * bz;
+        } else {
+            B = b[c] * b[c] + b[N + c] * b[N + c] + b[2 * N + c] * b[2 * N + c];
+        }
+        return cfl_speed<T>(u[c], u[N + c], u[2 * N + c], u[3 * N + c], u[4 * N + c], B);
+    }
+};
 
 // Block index -> tile index so that each XCD (blocks are dealt round-robin
 // over the 8 XCDs) receives a contiguous range of tiles: neighbouring tiles

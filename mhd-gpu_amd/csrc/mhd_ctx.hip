@@ -74,7 +74,7 @@ int soa_pad_z(int prec) { return env_pad("MHD_PAD_Z", 256 / prec, prec); }
 template <typename T> static hipError_t enqueue_dt(mhd_ctx *c)
 {
     return launch_dt_part<T>(c->cmax, (const T *)c->cfl_part, c->nparts, (T *)c->dt, (T *)c->t, c->t64, (T)c->tf,
-                             (T *)c->dts, c->iter, c->cap, c->st);
+                             (T *)c->dts, c->iter, c->cap, (const T *)c->u, (const T *)c->b, c->g, c->avg, c->st);
 }
 
 template <typename T> static hipError_t prime_cfl(mhd_ctx *c)

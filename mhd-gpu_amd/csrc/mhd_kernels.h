@@ -22,16 +22,17 @@ hipError_t launch_advect(const T *u, const T *bin, T *bout, int fr, int nx, int 
 template <typename T>
 hipError_t launch_cfl(const T *u, const T *b, int nx, int ny, int nz, int avg, unsigned long long *cmax,
                       hipStream_t st);
-// dt, t, dt history from *cmax; clears *cmax
-// dt from block maxima of a fused CFL (part) and the accumulator; clears the accumulator
+// dt from the block maxima of a fused CFL (part) and the accumulator, NaN
+// maxima resolved over the SoA state (u, b, g, avg) as the reference's
+// sequential max does; clears the accumulator
 template <typename T>
 hipError_t launch_dt_part(unsigned long long *cmax, const T *part, int np, T *dt, T *t, double *t64, T tf, T *dts,
-                          int *iter, int cap, hipStream_t st);
+                          int *iter, int cap, const T *u, const T *b, const SoaGrid &g, int avg, hipStream_t st);
 template <typename T> hipError_t launch_fill_zero(T *p, int n, hipStream_t st);
+// *out = 1/c from the accumulator (cleared), NaN maxima resolved over the AoS state
 template <typename T>
-hipError_t launch_dt(unsigned long long *cmax, T *dt, T *t, double *t64, T tf, T *dts, int *iter, int cap,
-                     hipStream_t st);
-template <typename T> hipError_t launch_cfl_value(unsigned long long *cmax, T *out, hipStream_t st);
+hipError_t launch_cfl_value(unsigned long long *cmax, T *out, const T *u, const T *b, int nx, int ny, int nz, int avg,
+                            hipStream_t st);
 template <typename T> hipError_t prepare_sweeps(int nx, int ny, int nz);
 hipError_t launch_zero_u64(unsigned long long *p, int n, hipStream_t st);
 // plain device copy kernel (bytes a multiple of 16)

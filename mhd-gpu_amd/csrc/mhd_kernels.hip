@@ -351,48 +351,28 @@ __global__ void __launch_bounds__(256) cfl_kernel(const T *__restrict__ u, const
                                                   int nz, unsigned long long *cmax)
 {
     const long ncell = (long)nx * ny * nz;
+    const AosSpeed<T, AVG> speed{u, b, nx, ny, nz};
     T c = (T)0;
-    for (long s = (long)blockIdx.x * blockDim.x + threadIdx.x; s < ncell; s += (long)gridDim.x * blockDim.x) {
-        const T *pu = u + 5 * s;
-        const T *pb = b + 3 * s;
-        T B;
-        if constexpr (AVG) {
-            const int i = (int)(s % nx);
-            const long r = s / nx;
-            const int j = (int)(r % ny), k = (int)(r / ny);
-            const long sip = s - i + (i + 1 == nx ? 0 : i + 1);
-            const long sjp = ((long)k * ny + (j + 1 == ny ? 0 : j + 1)) * nx + i;
-            const long skp = ((long)(k + 1 == nz ? 0 : k + 1) * ny + j) * nx + i;
-            const T bx = (T)((double)(pb[0] + b[3 * sip + 0]) / 2.0);
-            const T by = (T)((double)(pb[1] + b[3 * sjp + 1]) / 2.0);
-            const T bz = (T)((double)(pb[2] + b[3 * skp + 2]) / 2.0);
-            B = bx * bx + by * by + bz * bz;
-        } else {
-            B = pb[0] * pb[0] + pb[1] * pb[1] + pb[2] * pb[2];
-        }
-        const T t = cfl_speed<T>(pu[0], pu[1], pu[2], pu[3], pu[4], B);
-        c = c > t ? c : t;  // max_2num(c, t)
-    }
-    c = wave_max(c);
+    for (long s = (long)blockIdx.x * blockDim.x + threadIdx.x; s < ncell; s += (long)gridDim.x * blockDim.x)
+        c = cfl_max(c, speed(s));  // max_2num(c, t), NaN-sticky
+    c = wave_cfl_max(c);
     __shared__ T red[4];
     if (lane_id() == 0) red[threadIdx.x >> 6] = c;
     __syncthreads();
     if (threadIdx.x == 0) {
         T m = red[0];
-        for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = fmax(m, red[i]);
+        for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = cfl_max(m, red[i]);
         atomicMax(cmax, tobits(m));
     }
 }
 
-// dt update on the device (Heterogeneous_MHD/C/mhd.c:63-65): cfl = 1/c in
-// double rounded to T, dt = 0.9*cfl in double rounded to T, dt clamped to
-// (tf-t)/2, t += 2*dt (in T; an FP64 copy of t is accumulated beside it).
-// The max accumulator is cleared for the next step.
-template <typename T>
-__global__ void dt_kernel(unsigned long long *cmax, T *dt, T *t, double *t64, T tf, T *dts, int *iter, int cap)
+// The step's dt on the device (Heterogeneous_MHD/C/mhd.c:63-65) from the CFL
+// max: cfl = 1/c in double rounded to T, dt = 0.9*cfl in double rounded to
+// T, dt clamped to (tf-t)/2, t += 2*dt (in T; an FP64 copy of t is
+// accumulated beside it), dt recorded in the history.
+template <typename T> __device__ __forceinline__ void dt_update(T c, T *dt, T *t, double *t64, T tf, T *dts, int *iter,
+                                                                int cap)
 {
-    const T c = frombits<T>(*cmax);
-    *cmax = 0ull;
     const T cfl = (T)(1.0 / (double)c);
     T d = (T)(0.9 * (double)cfl);
     if (d > (tf - *t) / (T)2) d = (tf - *t) / (T)2;
@@ -404,42 +384,47 @@ __global__ void dt_kernel(unsigned long long *cmax, T *dt, T *t, double *t64, T 
     *iter = it + 1;
 }
 
-// The same dt when the CFL max comes as block maxima (part[0..np)) of the
-// previous step's fused last sweep, beside the accumulator (which a
-// standalone CFL fills after the state was set from outside).  Deterministic:
-// max is order-independent.
+// The dt of a device-resident box.  The CFL max comes as block maxima
+// (part[0..np)) of the previous step's fused last sweep, beside the
+// accumulator, which a standalone CFL fills after the state was set from
+// outside; both are NaN-sticky (cfl_max), and a NaN max is replaced by the
+// reference's sequential one over the state (cfl_after_last_nan).  Clears
+// the accumulator.  Deterministic: the max is order-independent.
 template <typename T>
 __global__ void __launch_bounds__(1024) dt_part_kernel(unsigned long long *cmax, const T *__restrict__ part, int np,
-                                                       T *dt, T *t, double *t64, T tf, T *dts, int *iter, int cap)
+                                                       T *dt, T *t, double *t64, T tf, T *dts, int *iter, int cap,
+                                                       const T *__restrict__ u, const T *__restrict__ b, SoaGrid g,
+                                                       int avg)
 {
     // four independent loads in flight per thread
     T c0 = (T)0, c1 = (T)0, c2 = (T)0, c3 = (T)0;
     const int nt = blockDim.x;
     int i = threadIdx.x;
     for (; i + 3 * nt < np; i += 4 * nt) {
-        c0 = fmax(c0, part[i]);
-        c1 = fmax(c1, part[i + nt]);
-        c2 = fmax(c2, part[i + 2 * nt]);
-        c3 = fmax(c3, part[i + 3 * nt]);
+        c0 = cfl_max(c0, part[i]);
+        c1 = cfl_max(c1, part[i + nt]);
+        c2 = cfl_max(c2, part[i + 2 * nt]);
+        c3 = cfl_max(c3, part[i + 3 * nt]);
     }
-    for (; i < np; i += nt) c0 = fmax(c0, part[i]);
-    T c = wave_max(fmax(fmax(c0, c1), fmax(c2, c3)));
+    for (; i < np; i += nt) c0 = cfl_max(c0, part[i]);
+    T c = wave_cfl_max(cfl_max(cfl_max(c0, c1), cfl_max(c2, c3)));
     __shared__ T red[16];
     if (lane_id() == 0) red[threadIdx.x >> 6] = c;
     __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < nt / 64; w++) c = cfl_max(c, red[w]);
+        red[0] = cfl_max(c, speed_from_bits<T>(*cmax));
+    }
+    __syncthreads();
+    c = red[0];
+    if (c != c) {
+        const long ncell = (long)g.nx * g.ny * g.nz;
+        c = avg ? cfl_after_last_nan<T>(ncell, SoaSpeed<T, true>{u, b, g})
+                : cfl_after_last_nan<T>(ncell, SoaSpeed<T, false>{u, b, g});
+    }
     if (threadIdx.x != 0) return;
-    for (int w = 1; w < nt / 64; w++) c = fmax(c, red[w]);
-    c = fmax(c, frombits<T>(*cmax));
     *cmax = 0ull;
-    const T cfl = (T)(1.0 / (double)c);
-    T d = (T)(0.9 * (double)cfl);
-    if (d > (tf - *t) / (T)2) d = (tf - *t) / (T)2;
-    *t = *t + (T)2 * d;
-    *t64 = *t64 + 2.0 * (double)d;
-    *dt = d;
-    const int it = *iter;
-    if (it < cap) dts[it] = d;
-    *iter = it + 1;
+    dt_update<T>(c, dt, t, t64, tf, dts, iter, cap);
 }
 
 template <typename T> __global__ void __launch_bounds__(256) fill_zero_kernel(T *p, int n)
@@ -447,10 +432,19 @@ template <typename T> __global__ void __launch_bounds__(256) fill_zero_kernel(T 
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = (T)0;
 }
 
-// cfl value 1/c (calcfl.c:61) for the step-level entry point
-template <typename T> __global__ void cfl_value_kernel(unsigned long long *cmax, T *out)
+// cfl value 1/c (calcfl.c:61) for the step-level entry point; a NaN max is
+// replaced by the reference's sequential one over the AoS state
+template <typename T>
+__global__ void __launch_bounds__(1024) cfl_value_kernel(unsigned long long *cmax, T *out, const T *__restrict__ u,
+                                                         const T *__restrict__ b, int nx, int ny, int nz, int avg)
 {
-    const T c = frombits<T>(*cmax);
+    T c = frombits<T>(*cmax);
+    if (c != c) {
+        const long ncell = (long)nx * ny * nz;
+        c = avg ? cfl_after_last_nan<T>(ncell, AosSpeed<T, true>{u, b, nx, ny, nz})
+                : cfl_after_last_nan<T>(ncell, AosSpeed<T, false>{u, b, nx, ny, nz});
+    }
+    if (threadIdx.x != 0) return;
     *cmax = 0ull;
     *out = (T)(1.0 / (double)c);
 }
@@ -753,18 +747,11 @@ hipError_t launch_cfl(const T *u, const T *b, int nx, int ny, int nz, int avg, u
 }
 
 template <typename T>
-hipError_t launch_dt(unsigned long long *cmax, T *dt, T *t, double *t64, T tf, T *dts, int *iter, int cap,
-                     hipStream_t st)
-{
-    hipLaunchKernelGGL(dt_kernel<T>, dim3(1), dim3(1), 0, st, cmax, dt, t, t64, tf, dts, iter, cap);
-    return hipGetLastError();
-}
-
-template <typename T>
 hipError_t launch_dt_part(unsigned long long *cmax, const T *part, int np, T *dt, T *t, double *t64, T tf, T *dts,
-                          int *iter, int cap, hipStream_t st)
+                          int *iter, int cap, const T *u, const T *b, const SoaGrid &g, int avg, hipStream_t st)
 {
-    hipLaunchKernelGGL(dt_part_kernel<T>, dim3(1), dim3(1024), 0, st, cmax, part, np, dt, t, t64, tf, dts, iter, cap);
+    hipLaunchKernelGGL(dt_part_kernel<T>, dim3(1), dim3(1024), 0, st, cmax, part, np, dt, t, t64, tf, dts, iter, cap,
+                       u, b, g, avg);
     return hipGetLastError();
 }
 
@@ -775,9 +762,11 @@ template <typename T> hipError_t launch_fill_zero(T *p, int n, hipStream_t st)
     return hipGetLastError();
 }
 
-template <typename T> hipError_t launch_cfl_value(unsigned long long *cmax, T *out, hipStream_t st)
+template <typename T>
+hipError_t launch_cfl_value(unsigned long long *cmax, T *out, const T *u, const T *b, int nx, int ny, int nz, int avg,
+                            hipStream_t st)
 {
-    hipLaunchKernelGGL(cfl_value_kernel<T>, dim3(1), dim3(1), 0, st, cmax, out);
+    hipLaunchKernelGGL(cfl_value_kernel<T>, dim3(1), dim3(1024), 0, st, cmax, out, u, b, nx, ny, nz, avg);
     return hipGetLastError();
 }
 
@@ -832,11 +821,11 @@ hipError_t launch_monitors(const T *u, const T *b, int nx, int ny, int nz, doubl
     template hipError_t launch_fluidx<T>(T *, const T *, int, int, int, int, const T *, hipStream_t);              \
     template hipError_t launch_advect<T>(const T *, const T *, T *, int, int, int, int, const T *, hipStream_t);              \
     template hipError_t launch_cfl<T>(const T *, const T *, int, int, int, int, unsigned long long *, hipStream_t); \
-    template hipError_t launch_dt<T>(unsigned long long *, T *, T *, double *, T, T *, int *, int, hipStream_t);    \
     template hipError_t launch_dt_part<T>(unsigned long long *, const T *, int, T *, T *, double *, T, T *, int *,   \
-                                          int, hipStream_t);                                                        \
+                                          int, const T *, const T *, const SoaGrid &, int, hipStream_t);            \
     template hipError_t launch_fill_zero<T>(T *, int, hipStream_t);                                                 \
-    template hipError_t launch_cfl_value<T>(unsigned long long *, T *, hipStream_t);                               \
+    template hipError_t launch_cfl_value<T>(unsigned long long *, T *, const T *, const T *, int, int, int, int,     \
+                                            hipStream_t);                                                           \
     template hipError_t launch_transpose<T>(T *, T *, const T *, const T *, int, int, int, int, hipStream_t);       \
     template hipError_t launch_monitors<T>(const T *, const T *, int, int, int, double *, int, double *, hipStream_t); \
     template hipError_t prepare_sweeps<T>(int, int, int);

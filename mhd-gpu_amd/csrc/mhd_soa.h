@@ -14,13 +14,6 @@ struct FastDiv {
 };
 FastDiv make_fastdiv(int d);
 
-// SoA layout of one box: cell (i,j,k) at i + j*sy + k*sz of each plane,
-// planes N apart; sy >= nx and sz >= sy*ny leave padding that breaks
-// power-of-two strides
-struct SoaGrid {
-    int nx, ny, nz;
-    long sy, sz, N;
-};
 SoaGrid make_grid(int nx, int ny, int nz, int pad_y, int pad_z);
 
 // one box in one sweep frame

@@ -112,13 +112,13 @@ __global__ void __launch_bounds__(256) cfl_soa_kernel(const T *__restrict__ u, c
 {
     T c = cfl_range<T, AVG>(u, b, g, (long)blockIdx.x * blockDim.x + threadIdx.x, cfl_units<T>(g),
                             (long)gridDim.x * blockDim.x);
-    c = wave_max(c);
+    c = wave_cfl_max(c);
     __shared__ T red[4];
     if (lane_id() == 0) red[threadIdx.x >> 6] = c;
     __syncthreads();
     if (threadIdx.x == 0) {
         T m = red[0];
-        for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = fmax(m, red[i]);
+        for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = cfl_max(m, red[i]);
         atomicMax(cmax, soa_bits(m));
     }
 }

@@ -108,14 +108,14 @@ __device__ __forceinline__ T cfl_range(const T *__restrict__ u, const T *__restr
                 }
                 const T sp = cfl_speed<T>(vget<T>(U[0], q), vget<T>(U[1], q), vget<T>(U[2], q), vget<T>(U[3], q),
                                           vget<T>(U[4], q), B);
-                c = c > sp ? c : sp;
+                c = cfl_max(c, sp);
             }
         } else {
             const int i = iu;
             const long s = row + i;
             const T sp = cfl_speed_at<T, AVG>(u, b, g, s, b[s], b[N + s], b[2 * N + s],
                                               row + (i + 1 == g.nx ? 0 : i + 1), rowjp + i, rowkp + i);
-            c = c > sp ? c : sp;
+            c = cfl_max(c, sp);
         }
     }
     return c;
@@ -455,15 +455,15 @@ __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__re
                     B = b1 * b1 + b2 * b2 + b3 * b3;
                 }
                 const T sp = cfl_speed<T>(ui[0], ui[n], ui[2 * n], ui[3 * n], ui[4 * n], B);
-                c = c > sp ? c : sp;  // max_2num (calcfl.c:55)
+                c = cfl_max(c, sp);  // max_2num (calcfl.c:55)
             }
         }
-        c = wave_max(c);
+        c = wave_cfl_max(c);
         if (lane_id() == 0) cfl_red_slot<T>()[w] = c;
         __syncthreads();
         if (threadIdx.x == 0) {
             T m = cfl_red_slot<T>()[0];
-            for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = fmax(m, cfl_red_slot<T>()[i]);
+            for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = cfl_max(m, cfl_red_slot<T>()[i]);
             cfl_part[tile] = m;
         }
     }
@@ -990,15 +990,15 @@ __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__rest
                     B = b1[q] * b1[q] + b2[q] * b2[q] + b3[q] * b3[q];
                 }
                 const T sp = cfl_speed<T>(um[0][q], um[1][q], um[2][q], um[3][q], um[4][q], B);
-                c = c > sp ? c : sp;  // max_2num (calcfl.c:55)
+                c = cfl_max(c, sp);  // max_2num (calcfl.c:55)
             }
         }
-        c = wave_max(c);
+        c = wave_cfl_max(c);
         if (lane_id() == 0) cfl_red_slot<T>()[w] = c;
         __syncthreads();
         if (threadIdx.x == 0) {
             T m = cfl_red_slot<T>()[0];
-            for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = fmax(m, cfl_red_slot<T>()[i]);
+            for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = cfl_max(m, cfl_red_slot<T>()[i]);
             // the block max: into the tile's slot, or (FP32, cmax given) into
             // the box's accumulator as float bits (all speeds >= 0: bit
             // order is value order)

@@ -293,50 +293,63 @@ void FN(mhd_oracle_transpose13)(R *uo, R *bo, const R *u, const R *b, int nx, in
 /* calcfl: 1/max over cells of max|v| + sqrt(|2B^2 + gamma p|/rho), with the
  * face values of B at (i,j,k) (Heterogeneous_MHD/C/calcfl.c:13-65; the cell
  * averages bx,by,bz computed at :38-43 are unused there).  avg_b != 0 selects
- * the cell-averaged B^2 of OpenMP/mhd.f90:112-120 and HIP/cu/cuda_cfl.cu:100-115.
- * Returns 1/c computed in double and rounded to R (calcfl.c:61). */
+ * those averages of calcfl.c:38-43 as B^2 (the choice of HIP/cu/cuda_cfl.cu:
+ * 100-115 and OpenMP/mhd.f90:112-120, here with the C path's arithmetic).
+ * The max is the reference's running c = max_2num(c, t) from c = 0 in
+ * (k,j,i) order (calcfl.c:26-57, :93-103), NaN semantics included: a NaN t
+ * makes c NaN and the next t replaces it.  Each plane k runs that recursion
+ * from its first cell, recording whether it met a NaN; the planes are then
+ * chained in order (a plane with a NaN sets c to its own tail value, any
+ * other plane p gives max_2num(c, max of p)), which is the sequential result
+ * whatever the thread count.  Returns 1/c computed in double and rounded to
+ * R (calcfl.c:61). */
 R FN(mhd_oracle_calcfl)(const R *u, const R *b, int nx, int ny, int nz, int avg_b)
 {
     const R gamma = (R)(5.0 / 3.0);
-    R c = (R)0.0;
-#pragma omp parallel
-    {
-        R cl = (R)0.0;
-#pragma omp for schedule(static)
-        for (int k = 0; k < nz; k++)
-            for (int j = 0; j < ny; j++)
-                for (int i = 0; i < nx; i++) {
-                    const size_t s = ((size_t)k * ny + j) * nx + i;
-                    const R u1 = u[5 * s], u2 = u[5 * s + 1], u3 = u[5 * s + 2], u4 = u[5 * s + 3],
-                            u5 = u[5 * s + 4];
-                    R B;
-                    if (avg_b) {
-                        const size_t sip = ((size_t)k * ny + j) * nx + (i + 1) % nx;
-                        const size_t sjp = ((size_t)k * ny + (j + 1) % ny) * nx + i;
-                        const size_t skp = ((size_t)((k + 1) % nz) * ny + j) * nx + i;
-                        /* calcfl.c:39-43: float sum, halved in double */
-                        const R bx = (R)((double)(b[3 * s + 0] + b[3 * sip + 0]) / 2.0);
-                        const R by = (R)((double)(b[3 * s + 1] + b[3 * sjp + 1]) / 2.0);
-                        const R bz = (R)((double)(b[3 * s + 2] + b[3 * skp + 2]) / 2.0);
-                        B = bx * bx + by * by + bz * bz;
-                    } else {
-                        B = b[3 * s] * b[3 * s] + b[3 * s + 1] * b[3 * s + 1] + b[3 * s + 2] * b[3 * s + 2];
-                    }
-                    /* calcfl.c:45-48 */
-                    const R t1 = (R)fabs((double)(u2 / u1)), t2 = (R)fabs((double)(u3 / u1)),
-                            t3 = (R)fabs((double)(u4 / u1));
-                    const R v = t1 > t2 ? (t1 > t3 ? t1 : t3) : (t2 > t3 ? t2 : t3);
-                    const R S = u2 * u2 + u3 * u3 + u4 * u4;
-                    const R ps = (R)(((double)u5 - (double)(S / u1) / 2.0) * ((double)gamma - 1.0) +
-                                     (2.0 - (double)gamma) * (double)B / 2.0); /* calcfl.c:50 */
-                    const R p = (R)((double)ps - (double)B / 2.0);             /* calcfl.c:52 */
-                    /* calcfl.c:54: v + sqrt(fabs(B*2.0 + gamma*p)/u1), in double */
-                    const R t = (R)((double)v + sqrt(fabs((double)B * 2.0 + (double)(gamma * p)) / (double)u1));
-                    cl = cl > t ? cl : t;                                      /* max_2num */
+    R *plane_c = (R *)malloc(sizeof(R) * (size_t)nz);
+    int *plane_nan = (int *)malloc(sizeof(int) * (size_t)nz);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; k++) {
+        R cl = (R)-INFINITY;
+        int seen = 0;
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const size_t s = ((size_t)k * ny + j) * nx + i;
+                const R u1 = u[5 * s], u2 = u[5 * s + 1], u3 = u[5 * s + 2], u4 = u[5 * s + 3], u5 = u[5 * s + 4];
+                R B;
+                if (avg_b) {
+                    const size_t sip = ((size_t)k * ny + j) * nx + (i + 1) % nx;
+                    const size_t sjp = ((size_t)k * ny + (j + 1) % ny) * nx + i;
+                    const size_t skp = ((size_t)((k + 1) % nz) * ny + j) * nx + i;
+                    /* calcfl.c:39-43: float sum, halved in double */
+                    const R bx = (R)((double)(b[3 * s + 0] + b[3 * sip + 0]) / 2.0);
+                    const R by = (R)((double)(b[3 * s + 1] + b[3 * sjp + 1]) / 2.0);
+                    const R bz = (R)((double)(b[3 * s + 2] + b[3 * skp + 2]) / 2.0);
+                    B = bx * bx + by * by + bz * bz;
+                } else {
+                    B = b[3 * s] * b[3 * s] + b[3 * s + 1] * b[3 * s + 1] + b[3 * s + 2] * b[3 * s + 2];
                 }
-#pragma omp critical
-        c = c > cl ? c : cl;
+                /* calcfl.c:45-48 */
+                const R t1 = (R)fabs((double)(u2 / u1)), t2 = (R)fabs((double)(u3 / u1)),
+                        t3 = (R)fabs((double)(u4 / u1));
+                const R v = t1 > t2 ? (t1 > t3 ? t1 : t3) : (t2 > t3 ? t2 : t3); /* max_3num */
+                const R S = u2 * u2 + u3 * u3 + u4 * u4;
+                const R ps = (R)(((double)u5 - (double)(S / u1) / 2.0) * ((double)gamma - 1.0) +
+                                 (2.0 - (double)gamma) * (double)B / 2.0); /* calcfl.c:50 */
+                const R p = (R)((double)ps - (double)B / 2.0);             /* calcfl.c:52 */
+                /* calcfl.c:54: v + sqrt(fabs(B*2.0 + gamma*p)/u1), in double */
+                const R t = (R)((double)v + sqrt(fabs((double)B * 2.0 + (double)(gamma * p)) / (double)u1));
+                if (t != t) seen = 1;
+                cl = cl > t ? cl : t; /* max_2num */
+            }
+        plane_c[k] = cl;
+        plane_nan[k] = seen;
     }
+    R c = (R)0.0;
+    if (nx > 0 && ny > 0)
+        for (int k = 0; k < nz; k++) c = plane_nan[k] ? plane_c[k] : (c > plane_c[k] ? c : plane_c[k]);
+    free(plane_c);
+    free(plane_nan);
     return (R)(1.0 / (double)c);
 }
 

@@ -19,6 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 from oracle_lib import Oracle, Reference  # noqa: E402
 from golden.states import random_state  # noqa: E402
+from golden.special_states import CFL_CASES, special_state  # noqa: E402
 
 
 def main():
@@ -62,6 +63,16 @@ def main():
     ua, ba = u0.copy(), b0.copy()
     Rf.transpose13(ua, ba, 8, 8, 8)
     out["rnd8_t13_u"], out["rnd8_t13_b"] = ua, ba
+
+    # non-positive densities, NaN and infinities: the CFL max's NaN semantics
+    # (calcfl.c:55) and the ranges beyond the GPU's fast-division window
+    for name in CFL_CASES:
+        u0, b0 = special_state(name, 12, 6, 10, seed=5)
+        out[f"spc_{name}_calcfl"] = np.float32(Rf.calcfl(u0, b0, 12, 6, 10))
+    for name in ("neg_density", "neg_density_many", "tiny_density", "huge_momentum"):
+        ua, ba = special_state(name, 10, 10, 10, seed=5)
+        out[f"spc_{name}_run3_dt"] = Rf.run(ua, ba, 10, 10, 10, 3)
+        out[f"spc_{name}_run3_u"], out[f"spc_{name}_run3_b"] = ua, ba
 
     np.savez_compressed(os.path.join(HERE, "golden_ref.npz"), **out)
     print("wrote", sorted(out))
