@@ -24,7 +24,9 @@ extern "C" {
 
 #define MHD_OK 0
 #define MHD_ERR_ARG (-1)      /* bad pointer or argument */
-#define MHD_ERR_SHAPE (-2)    /* unsupported grid shape (pencil length n needs n%K==0, 64*K>=n, K<=8) */
+#define MHD_ERR_SHAPE (-2)    /* unsupported grid shape: pencil lengths 2..1024 (FP64 2..512), as the
+                                 reference's one block of n threads; the batched and served forms need
+                                 n = L*K with L <= 64, K <= 8 */
 #define MHD_ERR_HIP (-3)      /* HIP runtime error (message in mhd_last_error) */
 #define MHD_ERR_NOMEM (-4)    /* device allocation failed or budget exceeded */
 #define MHD_ERR_STATE (-5)    /* call not valid in the object's current state */

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "mhd_kernels.h"
+#include "mhd_generic.h"
 #include "mhd_internal.h"
 
 namespace mhd {
@@ -34,28 +35,47 @@ int hip_fail(hipError_t e, const char *where)
     return fail(MHD_ERR_HIP, "%s: %s", where, hipGetErrorString(e));
 }
 
-int check_shape(int nx, int ny, int nz)
+// a pencil length the sweeps take in precision prec: the fast kernels'
+// n = L*K (K <= 8), else the generic ones' 2 <= n <= 1024 (FP64: 512)
+static bool pencil_ok(int n, int prec) { return pick_k(n) || gen_k(n, prec); }
+
+int check_shape(int nx, int ny, int nz, int prec)
 {
     if (nx < 2 || ny < 1 || nz < 1) return fail(MHD_ERR_SHAPE, "grid %dx%dx%d: need nx>=2, ny>=1, nz>=1", nx, ny, nz);
     if ((long)nx * ny * nz * 5 >= (1L << 31))
         return fail(MHD_ERR_SHAPE, "grid %dx%dx%d: 5*cells must stay below 2^31", nx, ny, nz);
-    if (!pick_k(nx)) return fail(MHD_ERR_SHAPE, "pencil length %d unsupported (need n%%K==0, 64*K>=n, K<=8)", nx);
+    if (!pencil_ok(nx, prec))
+        return fail(MHD_ERR_SHAPE, "pencil length %d unsupported (2..1024 cells; FP64 2..512)", nx);
     return MHD_OK;
 }
 
 // all three sweep directions of a box (the context path sweeps y and z too)
-int check_shape3(int nx, int ny, int nz)
+int check_shape3(int nx, int ny, int nz, int prec)
 {
-    int rc = check_shape(nx, ny, nz);
+    int rc = check_shape(nx, ny, nz, prec);
     if (rc) return rc;
     if (ny < 2 || nz < 2) return fail(MHD_ERR_SHAPE, "grid %dx%dx%d: the full step needs ny, nz >= 2", nx, ny, nz);
-    if (!pick_k(ny) || !pick_k(nz))
-        return fail(MHD_ERR_SHAPE, "grid %dx%dx%d: pencil length unsupported (need n%%K==0, 64*K>=n, K<=8)", nx, ny,
+    if (!pencil_ok(ny, prec) || !pencil_ok(nz, prec))
+        return fail(MHD_ERR_SHAPE, "grid %dx%dx%d: pencil length unsupported (2..1024 cells; FP64 2..512)", nx, ny,
                     nz);
     return MHD_OK;
 }
 
+int check_shape_fast3(int nx, int ny, int nz)
+{
+    int rc = check_shape3(nx, ny, nz, 4);
+    if (rc) return rc;
+    if (!pick_k(nx) || !pick_k(ny) || !pick_k(nz))
+        return fail(MHD_ERR_SHAPE,
+                    "grid %dx%dx%d: the batched and served forms need every pencil length n = L*K with K <= 8 and "
+                    "L <= 64 (the plain context takes any length up to 1024)",
+                    nx, ny, nz);
+    return MHD_OK;
+}
+
 // per-device scratch of the step-level entry points: the CFL max accumulator
+// (the caller holds g_scr_mu from the lookup until the accumulator is read
+// and cleared, so concurrent mhd_cfl calls do not share it)
 static std::mutex g_scr_mu;
 static unsigned long long *g_scr[64];
 
@@ -65,7 +85,6 @@ static int step_scratch(unsigned long long **p)
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
     if (dev < 0 || dev >= 64) return fail(MHD_ERR_ARG, "device %d out of range", dev);
-    std::lock_guard<std::mutex> lk(g_scr_mu);
     if (!g_scr[dev]) {
         e = hipMalloc((void **)&g_scr[dev], 64);
         if (e != hipSuccess) return fail(MHD_ERR_NOMEM, "hipMalloc scratch: %s", hipGetErrorString(e));
@@ -219,7 +238,7 @@ int mhd_mem_info(size_t *free_bytes, size_t *total_bytes)
 int mhd_fluidx(float *u, const float *b, const int *nx, const int *ny, const int *nz, const float *dt)
 {
     if (!u || !b || !nx || !ny || !nz || !dt) return fail(MHD_ERR_ARG, "mhd_fluidx: null argument");
-    int rc = check_shape(*nx, *ny, *nz);
+    int rc = check_shape(*nx, *ny, *nz, 4);
     if (rc) return rc;
     return sync_check(launch_fluidx<float>(u, b, 0, *nx, *ny, *nz, dt, 0), "mhd_fluidx");
 }
@@ -227,7 +246,7 @@ int mhd_fluidx(float *u, const float *b, const int *nx, const int *ny, const int
 int mhd_advectbyzx(const float *u, float *b, const int *nx, const int *ny, const int *nz, const float *dt)
 {
     if (!u || !b || !nx || !ny || !nz || !dt) return fail(MHD_ERR_ARG, "mhd_advectbyzx: null argument");
-    int rc = check_shape(*nx, *ny, *nz);
+    int rc = check_shape(*nx, *ny, *nz, 4);
     if (rc) return rc;
     const size_t bytes = (size_t)*nx * *ny * *nz * 3 * sizeof(float);
     std::lock_guard<std::mutex> lk(g_bcopy_mu);
@@ -242,7 +261,7 @@ int mhd_advectbyzx(const float *u, float *b, const int *nx, const int *ny, const
 int mhd_advectbyzx_noalloc(const float *u, float *b, const int *nx, const int *ny, const int *nz, const float *dt)
 {
     if (!u || !b || !nx || !ny || !nz || !dt) return fail(MHD_ERR_ARG, "mhd_advectbyzx_noalloc: null argument");
-    int rc = check_shape(*nx, *ny, *nz);
+    int rc = check_shape(*nx, *ny, *nz, 4);
     if (rc) return rc;
     const long cells = (long)*nx * *ny * *nz;
     if (cells > kNoAllocCells)
@@ -288,6 +307,7 @@ int mhd_cfl(const float *u, const float *b, const int *nx, const int *ny, const 
     if (!u || !b || !nx || !ny || !nz || !d_cfl) return fail(MHD_ERR_ARG, "mhd_cfl: null argument");
     if (*nx < 1 || *ny < 1 || *nz < 1) return fail(MHD_ERR_SHAPE, "mhd_cfl: bad grid");
     unsigned long long *scr = nullptr;
+    std::lock_guard<std::mutex> lk(g_scr_mu);
     int rc = step_scratch(&scr);
     if (rc) return rc;
     hipError_t e = launch_cfl<float>(u, b, *nx, *ny, *nz, avg_b, scr, 0);

@@ -672,7 +672,7 @@ void Engine::release_box(float *base, long N)
 
 int Engine::claim_slot(int nx, int ny, int nz, hipStream_t st, int *slot)
 {
-    int rc = check_shape3(nx, ny, nz);
+    int rc = check_shape_fast3(nx, ny, nz);
     if (rc) return rc;
     BoxDesc d;
     memset(&d, 0, sizeof d);
@@ -1491,7 +1491,7 @@ int mhd_batch_create(mhd_batch **out, int nbox, const int *dims, int avg_b)
     if (!out || !dims || nbox < 1) return fail(MHD_ERR_ARG, "mhd_batch_create: bad argument");
     *out = nullptr;
     for (int i = 0; i < nbox; i++) {
-        int rc = check_shape3(dims[3 * i], dims[3 * i + 1], dims[3 * i + 2]);
+        int rc = check_shape_fast3(dims[3 * i], dims[3 * i + 1], dims[3 * i + 2]);
         if (rc) return rc;
     }
     mhd_batch *bt = new mhd_batch();

@@ -306,6 +306,62 @@ __device__ __forceinline__ void shift_prev(const T (&a)[K], T (&o)[K], int src_p
     for (int q = 1; q < K; q++) o[q] = a[q - 1];
 }
 
+// ---- lane geometry of a pencil ----
+// Whole pencil: n = L*K cells on L lanes; next/prev are the neighbour lanes
+// (periodic over L).
+struct LanesFull {
+    int next, prev;
+    __device__ __forceinline__ bool valid(int) const { return true; }
+};
+// Ragged pencil: L = ceil(n/K) lanes, the last holding r = n - (L-1)*K < K
+// cells.  wq is the lane's last cell (K-1, or r-1 in the last lane); cells
+// beyond it hold values nothing valid reads (the shifts below wrap the
+// pencil at wq), and they are kept out of every pencil-wide reduction.
+struct LanesRagged {
+    int next, prev, wq;
+    __device__ __forceinline__ bool valid(int q) const { return q <= wq; }
+};
+__device__ __forceinline__ LanesRagged ragged_lanes(int n, int K, int lc)
+{
+    const int L = (n + K - 1) / K;
+    LanesRagged g;
+    g.next = lc + 1 == L ? 0 : lc + 1;
+    g.prev = lc == 0 ? L - 1 : lc - 1;
+    g.wq = lc == L - 1 ? n - (L - 1) * K - 1 : K - 1;
+    return g;
+}
+
+template <typename T, int K> __device__ __forceinline__ void shift_next(const T (&a)[K], T (&o)[K], const LanesFull &g)
+{
+    shift_next<T, K>(a, o, g.next);
+}
+template <typename T, int K> __device__ __forceinline__ void shift_prev(const T (&a)[K], T (&o)[K], const LanesFull &g)
+{
+    shift_prev<T, K>(a, o, g.prev);
+}
+// a[q] <- a[q+1]: the next lane's first cell lands at wq (lane 0's first cell
+// in the last lane, which is where the pencil wraps)
+template <typename T, int K>
+__device__ __forceinline__ void shift_next(const T (&a)[K], T (&o)[K], const LanesRagged &g)
+{
+    const T x = shfl_from(a[0], g.next);
+#pragma unroll
+    for (int q = 0; q < K - 1; q++) o[q] = q == g.wq ? x : a[q + 1];
+    o[K - 1] = x;
+}
+// a[q] <- a[q-1]: each lane sends its cell wq, so lane 0 receives the
+// pencil's last cell
+template <typename T, int K>
+__device__ __forceinline__ void shift_prev(const T (&a)[K], T (&o)[K], const LanesRagged &g)
+{
+    T s = a[K - 1];
+#pragma unroll
+    for (int q = 0; q < K - 1; q++) s = q == g.wq ? a[q] : s;
+    o[0] = shfl_from(s, g.prev);
+#pragma unroll
+    for (int q = 1; q < K; q++) o[q] = a[q - 1];
+}
+
 // Max over the 64 lanes (callers pass 0 from inactive lanes; all values >= 0).
 // fmax skips NaN, like find_max's strict > (mhdflux.c:66-69).
 template <typename T> __device__ __forceinline__ T wave_max(T x)

@@ -1,17 +1,23 @@
 // Per-pencil compute core of the sweeps, shared by every kernel layout:
-// a pencil of n = L*K cells lives in one wavefront, lane l < L holding cells
+// a pencil of n cells lives in one wavefront, lane l < L holding cells
 // l*K .. l*K+K-1 in registers; lanes >= L shadow lane l % L.  Neighbours along
 // the pencil come from lane shuffles, the pencil max from a wave reduction.
+// The lane geometry LN is LanesFull (n = L*K) or LanesRagged (the last lane
+// holds fewer than K cells; its other cells stay out of every reduction).
 #pragma once
+#include <type_traits>
+
 #include "mhd_common.h"
 
 namespace mhd {
 
 // Unnormalised fluxes v and the pencil's freezing speed c (mhdflux.c:21-69),
 // then v/c.
-template <typename T, int K>
-__device__ __forceinline__ void mhdflux_pencil(const T (&uu)[K][5], const T (&bx)[K][3], T (&v)[K][5], T &c)
+template <typename T, int K, class LN>
+__device__ __forceinline__ void mhdflux_pencil(const T (&uu)[K][5], const T (&bx)[K][3], T (&v)[K][5], T &c,
+                                               const LN &ln)
 {
+    constexpr bool whole = std::is_same<LN, LanesFull>::value;
     T cl = (T)0;
     if constexpr (sizeof(T) == 4) {
         // the divisions by u1 through one reciprocal per cell (exact while no
@@ -24,8 +30,21 @@ __device__ __forceinline__ void mhdflux_pencil(const T (&uu)[K][5], const T (&bx
 #pragma unroll
         for (int q = 0; q < K; q++) {
             T ci;
-            mhdflux_cell_bulk(uu[q], bx[q], v[q], ci, uok, vmax, emin);
-            cl = fmax(cl, ci);
+            if constexpr (whole) {
+                mhdflux_cell_bulk(uu[q], bx[q], v[q], ci, uok, vmax, emin);
+                cl = fmax(cl, ci);
+            } else {
+                bool ok1 = true;
+                float vm1 = 0.0f;
+                int em1 = 0;
+                mhdflux_cell_bulk(uu[q], bx[q], v[q], ci, ok1, vm1, em1);
+                if (ln.valid(q)) {
+                    uok = uok && ok1;
+                    vmax = fmaxf(vmax, vm1);
+                    emin = min(emin, em1);
+                    cl = fmax(cl, ci);
+                }
+            }
         }
         if (!__all(uok && vmax < 0x1p60f && emin >= -60)) {
             cl = (T)0;
@@ -33,7 +52,7 @@ __device__ __forceinline__ void mhdflux_pencil(const T (&uu)[K][5], const T (&bx
             for (int q = 0; q < K; q++) {
                 T ci;
                 mhdflux_cell<T>(uu[q], bx[q], v[q], ci);
-                cl = fmax(cl, ci);
+                if (ln.valid(q)) cl = fmax(cl, ci);
             }
         }
     } else {
@@ -41,7 +60,7 @@ __device__ __forceinline__ void mhdflux_pencil(const T (&uu)[K][5], const T (&bx
         for (int q = 0; q < K; q++) {
             T ci;
             mhdflux_cell<T>(uu[q], bx[q], v[q], ci);
-            cl = fmax(cl, ci);
+            if (ln.valid(q)) cl = fmax(cl, ci);
         }
     }
     c = wave_max(cl);
@@ -65,8 +84,10 @@ __device__ __forceinline__ void mhdflux_pencil(const T (&uu)[K][5], const T (&bx
             for (int q = 0; q < K; q++)
 #pragma unroll
                 for (int m = 0; m < 5; m++) {
-                    vmax = fmaxf(vmax, fabsf(v[q][m]));
-                    emin = min(emin, __builtin_amdgcn_frexp_expf(v[q][m]));
+                    if (ln.valid(q)) {
+                        vmax = fmaxf(vmax, fabsf(v[q][m]));
+                        emin = min(emin, __builtin_amdgcn_frexp_expf(v[q][m]));
+                    }
                 }
             if (__all(cok && vmax < 0x1p60f && emin >= -60)) {
                 const Recip rc = recip_of(c);
@@ -102,15 +123,15 @@ template <typename T, int K> __device__ __forceinline__ void cprod(T c, const T 
 }
 
 // tvd1 on a pencil held K cells per lane; uu is updated in place.
-template <typename T, int K>
-__device__ __forceinline__ void tvd1_pencil(T (&uu)[K][5], const T (&bx)[K][3], T dt, int src_next, int src_prev)
+template <typename T, int K, class LN>
+__device__ __forceinline__ void tvd1_pencil(T (&uu)[K][5], const T (&bx)[K][3], T dt, const LN &ln)
 {
     const T h = (T)(-1.0 * (double)dt / 2.0);
     const T mdt = (T)(-1.0 * (double)dt);
     T v[K][5], c;
     T u1[K][5];
     // first half step (tvd1.c:34-49)
-    mhdflux_pencil<T, K>(uu, bx, v, c);
+    mhdflux_pencil<T, K>(uu, bx, v, c, ln);
 #pragma unroll
     for (int m = 0; m < 5; m++) {
         T wr[K], wl[K], fr[K], wln[K], fl[K], fx[K], fxp[K];
@@ -119,17 +140,17 @@ __device__ __forceinline__ void tvd1_pencil(T (&uu)[K][5], const T (&bx)[K][3], 
             wr[q] = uu[q][m] + v[q][m];
             wl[q] = uu[q][m] - v[q][m];
         }
-        shift_next<T, K>(wl, wln, src_next);
+        shift_next<T, K>(wl, wln, ln);
         cprod<T, K>(c, wr, fr);
         cprod<T, K>(c, wln, fl);
 #pragma unroll
         for (int q = 0; q < K; q++) fx[q] = (T)0.5 * fr[q] + (T)(-0.5) * fl[q];
-        shift_prev<T, K>(fx, fxp, src_prev);
+        shift_prev<T, K>(fx, fxp, ln);
 #pragma unroll
         for (int q = 0; q < K; q++) u1[q][m] = uu[q][m] + h * (fx[q] - fxp[q]);
     }
     // full step with limited fluxes (tvd1.c:52-116)
-    mhdflux_pencil<T, K>(u1, bx, v, c);
+    mhdflux_pencil<T, K>(u1, bx, v, c, ln);
 #pragma unroll
     for (int m = 0; m < 5; m++) {
         // tvd1.c:59-90 forms dfrp = 0.5*fr[i+1] + (-0.5)*fr[i] and
@@ -145,7 +166,7 @@ __device__ __forceinline__ void tvd1_pencil(T (&uu)[K][5], const T (&bx)[K][3], 
             wr[q] = u1[q][m] + v[q][m];
             wl[q] = u1[q][m] - v[q][m];
         }
-        shift_next<T, K>(wl, wln, src_next);
+        shift_next<T, K>(wl, wln, ln);
         cprod<T, K>(c, wr, fr);
         cprod<T, K>(c, wln, fl);
 #pragma unroll
@@ -153,22 +174,22 @@ __device__ __forceinline__ void tvd1_pencil(T (&uu)[K][5], const T (&bx)[K][3], 
             hr[q] = (T)0.5 * fr[q];
             hl[q] = (T)0.5 * fl[q];
         }
-        shift_next<T, K>(hr, hrn, src_next);
-        shift_next<T, K>(hl, hln, src_next);
+        shift_next<T, K>(hr, hrn, ln);
+        shift_next<T, K>(hl, hln, ln);
 #pragma unroll
         for (int q = 0; q < K; q++) {
             dfrp[q] = hrn[q] - hr[q];
             dflp[q] = hl[q] - hln[q];
         }
-        shift_prev<T, K>(dfrp, dfrm, src_prev);
-        shift_prev<T, K>(dflp, dflm, src_prev);
+        shift_prev<T, K>(dfrp, dfrm, ln);
+        shift_prev<T, K>(dflp, dflm, ln);
 #pragma unroll
         for (int q = 0; q < K; q++) {
             const T dfr = vanleer<T>(dfrp[q], dfrm[q]);
             const T dfl = vanleer<T>(dflp[q], dflm[q]);
             fx[q] = (T)0.5 * (fr[q] - fl[q]) + (T)0.5 * (dfr - dfl);
         }
-        shift_prev<T, K>(fx, fxp, src_prev);
+        shift_prev<T, K>(fx, fxp, ln);
 #pragma unroll
         for (int q = 0; q < K; q++) uu[q][m] = uu[q][m] + mdt * (fx[q] - fxp[q]);
     }
@@ -176,29 +197,29 @@ __device__ __forceinline__ void tvd1_pencil(T (&uu)[K][5], const T (&bx)[K][3], 
 
 // tvdb on a pencil held K cells per lane (tvdb.c:10-91): bin -> bnew, and
 // F[i] = flux[i-1], the shifted flux the CT update uses.
-template <typename T, int K>
+template <typename T, int K, class LN>
 __device__ __forceinline__ void tvdb_pencil(const T (&bin)[K], const T (&vg)[K], T dt, T (&bnew)[K], T (&F)[K],
-                                            int src_next, int src_prev)
+                                            const LN &ln)
 {
     T vgn[K], vh[K], bv[K], bvn[K], f1[K], f1p[K], b1[K], g[K], gn[K], gnn[K], gp[K], fl[K], flp[K];
-    shift_next<T, K>(vg, vgn, src_next);
+    shift_next<T, K>(vg, vgn, ln);
 #pragma unroll
     for (int q = 0; q < K; q++) {
         vh[q] = (T)0.5 * vg[q] + (T)0.5 * vgn[q];
         bv[q] = bin[q] * vg[q];
     }
-    shift_next<T, K>(bv, bvn, src_next);
+    shift_next<T, K>(bv, bvn, ln);
 #pragma unroll
     for (int q = 0; q < K; q++) f1[q] = vh[q] > (T)0 ? bv[q] : bvn[q];
-    shift_prev<T, K>(f1, f1p, src_prev);
+    shift_prev<T, K>(f1, f1p, ln);
 #pragma unroll
     for (int q = 0; q < K; q++) {
         b1[q] = (T)((double)bin[q] - (double)((f1[q] - f1p[q]) * dt) / 2.0);
         g[q] = vg[q] * b1[q];
     }
-    shift_next<T, K>(g, gn, src_next);
-    shift_next<T, K>(gn, gnn, src_next);
-    shift_prev<T, K>(g, gp, src_prev);
+    shift_next<T, K>(g, gn, ln);
+    shift_next<T, K>(gn, gnn, ln);
+    shift_prev<T, K>(g, gp, ln);
 #pragma unroll
     for (int q = 0; q < K; q++) {
         T w, wp, wm;
@@ -214,12 +235,25 @@ __device__ __forceinline__ void tvdb_pencil(const T (&bin)[K], const T (&vg)[K],
         const T dw = vanleer<T>(wm, wp);
         fl[q] = (w + dw) * dt;
     }
-    shift_prev<T, K>(fl, flp, src_prev);
+    shift_prev<T, K>(fl, flp, ln);
 #pragma unroll
     for (int q = 0; q < K; q++) {
         bnew[q] = bin[q] - (fl[q] - flp[q]);
         F[q] = flp[q];
     }
+}
+
+// the same for a whole pencil with the neighbour lanes given as indices
+template <typename T, int K>
+__device__ __forceinline__ void tvd1_pencil(T (&uu)[K][5], const T (&bx)[K][3], T dt, int src_next, int src_prev)
+{
+    tvd1_pencil<T, K>(uu, bx, dt, LanesFull{src_next, src_prev});
+}
+template <typename T, int K>
+__device__ __forceinline__ void tvdb_pencil(const T (&bin)[K], const T (&vg)[K], T dt, T (&bnew)[K], T (&F)[K],
+                                            int src_next, int src_prev)
+{
+    tvdb_pencil<T, K>(bin, vg, dt, bnew, F, LanesFull{src_next, src_prev});
 }
 
 }  // namespace mhd

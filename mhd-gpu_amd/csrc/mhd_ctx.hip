@@ -105,15 +105,26 @@ template <typename T> static hipError_t enqueue_sweep(mhd_ctx *c, int s)
     const int fr = kSweepFrame[s];
     T *bcur = (T *)(field_buffer_before(s) ? c->b2 : c->b);
     T *bnext = (T *)(field_buffer_before(s) ? c->b : c->b2);
+    const int n = fr == 0 ? c->nx : fr == 1 ? c->ny : c->nz;
+    const bool generic = !pick_k(n);  // a pencil length only the generic kernels take
     if (kSweepOp[s] == 0) {
-        // the last sweep (fluidx, x frame) also forms the next step's CFL
+        // the last sweep (fluidx, x frame) also forms the next step's CFL; the
+        // generic kernels do not, so a standalone CFL of the new state follows
+        if (s == 11 && generic) {
+            hipError_t e = launch_fluidx_soa<T>((T *)c->u, bcur, fr, c->g, (const T *)c->dt, c->st);
+            if (e == hipSuccess)
+                e = launch_cfl_soa<T>((const T *)c->u, (const T *)bcur, c->g, c->avg, c->cmax, c->st);
+            return e;
+        }
         if (s == 11)
             return launch_fluidx_soa<T>((T *)c->u, bcur, fr, c->g, (const T *)c->dt, c->st, (T *)c->cfl_part,
                                         c->avg ? 2 : 1);
         return launch_fluidx_soa<T>((T *)c->u, bcur, fr, c->g, (const T *)c->dt, c->st);
     }
-    // x frame, and y/z frames of small boxes: LDS tiles; else column streaming
-    const bool tiles = c->adv_form == 0 ? !stream_pays(make_stream_box(fr, c->g)) : c->adv_form == 1;
+    // x frame, and y/z frames of small boxes or generic pencil lengths: LDS
+    // tiles; else column streaming
+    const bool tiles =
+        generic || (c->adv_form == 0 ? !stream_pays(make_stream_box(fr, c->g)) : c->adv_form == 1);
     if (fr == 0 || tiles)
         return launch_advect_soa<T>((const T *)c->u, bcur, bnext, fr, c->g, (const T *)c->dt, c->st);
     return launch_advect_stream<T>((const T *)c->u, bcur, bnext, (T *)c->fa, fr, c->g, (const T *)c->dt, c->st,
@@ -173,7 +184,7 @@ int mhd_ctx_create(mhd_ctx **out, int nx, int ny, int nz, int precision, int avg
     if (!out) return fail(MHD_ERR_ARG, "mhd_ctx_create: null out");
     *out = nullptr;
     if (precision != 4 && precision != 8) return fail(MHD_ERR_ARG, "precision must be 4 or 8");
-    int rc = check_shape3(nx, ny, nz);
+    int rc = check_shape3(nx, ny, nz, precision);
     if (rc) return rc;
     mhd_ctx *c = new mhd_ctx();
     memset(c, 0, sizeof *c);

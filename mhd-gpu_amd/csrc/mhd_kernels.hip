@@ -12,6 +12,7 @@
 // of the freezing speed is a wave reduction.
 #include "mhd_kernels.h"
 #include "mhd_core.h"
+#include "mhd_generic.h"
 
 namespace mhd {
 
@@ -647,6 +648,7 @@ template <typename T, int K, int FR> static hipError_t sweep_attr_both()
 template <typename T> hipError_t prepare_frame(int fr, int n)
 {
     const int K = pick_k(n);
+    if (!K) return prepare_gen<T>(n, 0);
 #define PA0(KK) sweep_attr_both<T, KK, 0>()
 #define PA1(KK) sweep_attr_both<T, KK, 1>()
 #define PA2(KK) sweep_attr_both<T, KK, 2>()
@@ -700,6 +702,7 @@ hipError_t launch_fluidx(T *u, const T *b, int fr, int nx, int ny, int nz, const
 {
     const Frame f = make_frame(fr, nx, ny, nz);
     const int K = pick_k(f.ni);
+    if (!K) return launch_fluidx_gen<T>(u, b, f, 0, 0, fr, dt, st);
 #define FX0(KK) fluidx_launch_k<T, KK, 0>(u, b, f, dt, st)
 #define FX1(KK) fluidx_launch_k<T, KK, 1>(u, b, f, dt, st)
 #define FX2(KK) fluidx_launch_k<T, KK, 2>(u, b, f, dt, st)
@@ -717,6 +720,7 @@ hipError_t launch_advect(const T *u, const T *bin, T *bout, int fr, int nx, int 
 {
     const Frame f = make_frame(fr, nx, ny, nz);
     const int K = pick_k(f.ni);
+    if (!K) return launch_advect_gen<T>(u, bin, bout, f, 0, 0, fr, dt, st);
 #define AX0(KK) advect_launch_k<T, KK, 0>(u, bin, bout, f, dt, st)
 #define AX1(KK) advect_launch_k<T, KK, 1>(u, bin, bout, f, dt, st)
 #define AX2(KK) advect_launch_k<T, KK, 2>(u, bin, bout, f, dt, st)

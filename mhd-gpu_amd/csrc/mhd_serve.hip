@@ -347,7 +347,7 @@ int mhd_server_submit(mhd_server *sv, int nx, int ny, int nz, int nsteps, const 
                       float *out_u, float *out_b, long long *req_id)
 {
     if (!sv || !h_u || !h_b || !out_u || !out_b || nsteps < 1) return fail(MHD_ERR_ARG, "mhd_server_submit: bad argument");
-    int rc = check_shape3(nx, ny, nz);
+    int rc = check_shape_fast3(nx, ny, nz);
     if (rc) return rc;
     Request *r = new Request();
     r->nx = nx;
@@ -360,6 +360,7 @@ int mhd_server_submit(mhd_server *sv, int nx, int ny, int nz, int nsteps, const 
     r->out_b = out_b;
     r->bytes = device_bytes(*r);
     r->arrived = Clock::now();
+    long long id = -1;
     {
         std::lock_guard<std::mutex> lk(sv->mu);
         if (sv->stop) {
@@ -371,15 +372,16 @@ int mhd_server_submit(mhd_server *sv, int nx, int ny, int nz, int nsteps, const 
             delete r;
             return fail(MHD_ERR_FULL, "mhd_server_submit: request needs %zu bytes, budget %zu", need, sv->budget);
         }
-        r->id = sv->next_id++;
+        r->id = id = sv->next_id++;
         if (!sv->any) {
             sv->first_arrival = r->arrived;
             sv->any = true;
         }
         sv->queue.push_back(r);
     }
+    // r may be finished and freed by another thread's poll from here on
     sv->cv.notify_all();
-    if (req_id) *req_id = r->id;
+    if (req_id) *req_id = id;
     return MHD_OK;
 }
 

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 
+#include "mhd_generic.h"
 #include "mhd_soa_stream.h"
 #include "mhd_soa_tiles.h"
 
@@ -342,6 +343,8 @@ hipError_t launch_fluidx_soa(T *u, const T *b, int fr, const SoaGrid &g, const T
     if (cflm && fr != 0) return hipErrorInvalidValue;
     const SoaBox bx = make_soa_box<T>(fr, g);
     const int K = pick_k(bx.f.ni);
+    // any other pencil length: the generic kernels (no fused CFL)
+    if (!K) return cflm ? hipErrorInvalidValue : launch_fluidx_gen<T>(u, b, bx.f, bx.N, 1, fr, dt, st);
 #define SF0(KK) fluidx_soa_k<T, KK, 0>(u, b, bx, dt, st, cfl_part, cflm)
 #define SF1(KK) fluidx_soa_k<T, KK, 1>(u, b, bx, dt, st, nullptr, 0)
 #define SF2(KK) fluidx_soa_k<T, KK, 2>(u, b, bx, dt, st, nullptr, 0)
@@ -359,6 +362,7 @@ hipError_t launch_advect_soa(const T *u, const T *bin, T *bout, int fr, const So
 {
     const SoaBox bx = make_soa_box<T>(fr, g);
     const int K = pick_k(bx.f.ni);
+    if (!K) return launch_advect_gen<T>(u, bin, bout, bx.f, bx.N, 1, fr, dt, st);
 #define SA0(KK) advect_soa_k<T, KK, 0>(u, bin, bout, bx, dt, st)
 #define SA1(KK) advect_soa_k<T, KK, 1>(u, bin, bout, bx, dt, st)
 #define SA2(KK) advect_soa_k<T, KK, 2>(u, bin, bout, bx, dt, st)
@@ -458,7 +462,7 @@ template <typename T> hipError_t prepare_soa(int nx, int ny, int nz)
         case 5: e = PS(5); break;
         case 6: e = PS(6); break;
         case 8: e = PS(8); break;
-        default: break;
+        default: e = prepare_gen<T>(n3[fr], 1); break;
         }
 #undef PS
         if (e != hipSuccess) return e;

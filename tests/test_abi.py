@@ -42,9 +42,14 @@ def test_argument_errors_are_codes_not_exits():
     assert L.mhd_fluidx(None, None, ctypes.byref(n), ctypes.byref(n), ctypes.byref(n), None) == -1
     assert "null" in L.mhd_last_error().decode()
     p = ctypes.c_void_p()
-    # pencil length 67 has no K with 67 % K == 0 and 64*K >= 67
-    assert L.mhd_ctx_create(ctypes.byref(p), 67, 8, 8, 4, 0) == -2
+    # pencils above 1024 cells (the reference's one-block limit), and FP64
+    # pencils above 512, are refused before any device work
+    assert L.mhd_ctx_create(ctypes.byref(p), 1025, 8, 8, 4, 0) == -2
+    assert L.mhd_ctx_create(ctypes.byref(p), 8, 600, 8, 8, 0) == -2
     assert L.mhd_ctx_create(ctypes.byref(p), 8, 8, 8, 3, 0) == -1
+    # the batched form takes only the fast kernels' lengths (n = L*K, K <= 8)
+    dims = (ctypes.c_int * 3)(67, 8, 8)
+    assert L.mhd_batch_create(ctypes.byref(p), 1, dims, 0) == -2
     with pytest.raises(mhdgpu.MHDError) as e:
         mhdgpu.init(os.path.join(GOLDEN, "does_not_exist.dat"), 16)
     assert e.value.code == -6

@@ -76,7 +76,10 @@ needs_ref = pytest.mark.skipif(not oracle_lib.reference_available(),
 
 @needs_ref
 @pytest.mark.parametrize("shape,seed,amp", [((16, 16, 16), 21, 0.3), ((20, 6, 10), 22, 0.45),
-                                            ((9, 5, 3), 23, 0.2), ((24, 24, 24), 24, 0.6)])
+                                            ((9, 5, 3), 23, 0.2), ((24, 24, 24), 24, 0.6),
+                                            # pencil lengths of the generic GPU sweeps, up to 1024
+                                            ((67, 5, 3), 25, 0.45), ((131, 4, 3), 26, 0.45),
+                                            ((600, 3, 2), 27, 0.4), ((1024, 2, 2), 28, 0.4)])
 def test_oracle_equals_reference_build_ops(oracle, shape, seed, amp):
     ref = oracle_lib.Reference()
     nx, ny, nz = shape
