@@ -159,6 +159,14 @@ int mhd_ctx_dts(mhd_ctx *ctx, double *h_dts, int max, int *count);
 int mhd_ctx_time(mhd_ctx *ctx, double *t, double *t64);
 /* FP64 monitors: out[6] = total mass, total energy, momentum x, y, z, max |div B| */
 int mhd_ctx_monitors(mhd_ctx *ctx, double *out6);
+/* FP64 monitors recorded inside the captured step after every step since
+ * the last reset (no counterpart in the reference, whose drivers print probe
+ * values per step, Heterogeneous_MHD/C/mhd.c:67, HIP/cu/cuda_main.cu:86):
+ * h_out[6*s .. 6*s+5] = the six values of mhd_ctx_monitors for the state
+ * after step s+1, summed in a fixed order (deterministic run to run).  At
+ * most max entries are copied (the device keeps the first 65536); *count =
+ * steps taken. */
+int mhd_ctx_monitor_history(mhd_ctx *ctx, double *h_out, int max, int *count);
 /* device pointers of the state, for zero-copy callers: the context keeps the
  * state as structure of arrays, d_u = 5 planes (rho, rho vx, rho vy, rho vz,
  * E) and d_b = 3 planes (bx, by, bz), each laid out as mhd_ctx_layout says */

@@ -440,6 +440,44 @@ template <typename T, class Speed> __device__ T cfl_after_last_nan(long ncell, c
     return r;
 }
 
+// FP64 monitors of a block of cells: the lanes' partials ms[0..4] (mass,
+// energy, momentum x, y, z) summed and ms[5] (max |div B|) max-ed, in a
+// fixed order - an xor butterfly in each wave (both lanes of a pair add the
+// same two values, so every lane ends with the same sum), then the waves in
+// index order - so the result does not depend on timing.  Thread 0 writes
+// dst[0..5].  Every thread of the block calls it.
+__device__ __forceinline__ void monitor_block_store(const double (&ms)[6], double *dst)
+{
+    __shared__ double red[16][6];
+    double a[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) a[q] = ms[q];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const int src = lane_id() ^ off;
+#pragma unroll
+        for (int q = 0; q < 5; q++) a[q] = a[q] + shfl_from(a[q], src);
+        a[5] = fmax(a[5], shfl_from(a[5], src));
+    }
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0)
+#pragma unroll
+        for (int q = 0; q < 6; q++) red[w][q] = a[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s[6];
+#pragma unroll
+        for (int q = 0; q < 6; q++) s[q] = red[0][q];
+        for (int v = 1; v < (int)(blockDim.x >> 6); v++) {
+#pragma unroll
+            for (int q = 0; q < 5; q++) s[q] = s[q] + red[v][q];
+            s[5] = fmax(s[5], red[v][5]);
+        }
+#pragma unroll
+        for (int q = 0; q < 6; q++) dst[q] = s[q];
+    }
+}
+
 // The CFL speed (calcfl.c:45-54) of the cell of linear index
 // s = (k*ny+j)*nx+i, with the face B at the cell or (AVG) the cell average
 // of calcfl.c:38-43: over a box in the step-level AoS layout, and over a

@@ -38,6 +38,8 @@ struct mhd_ctx {
     int *iter;
     double *mon_part, *mon_out;
     int mon_nblk;
+    double *mon_tile;          // per-tile monitor partials of the step's last sweep (6 per x-frame tile)
+    double *mon_hist;          // FP64 monitors after each step (6 per step, kMonCap steps)
     double tf;
     hipGraph_t graph;
     hipGraphExec_t exec;
@@ -47,6 +49,7 @@ struct mhd_ctx {
 };
 
 static const int kDtCap = 1 << 20;
+static const int kMonCap = 1 << 16;
 
 // Padding of the SoA rows and planes (in cells).  A plane of 256x256 floats
 // is 256 KiB: z-sweep tiles read rows one plane apart, and at a power-of-two
@@ -110,15 +113,26 @@ template <typename T> static hipError_t enqueue_sweep(mhd_ctx *c, int s)
     if (kSweepOp[s] == 0) {
         // the last sweep (fluidx, x frame) also forms the next step's CFL; the
         // generic kernels do not, so a standalone CFL of the new state follows
+        // (and a standalone monitor pass); then the FP64 monitors of the new
+        // state are reduced into the step's history entry
         if (s == 11 && generic) {
             hipError_t e = launch_fluidx_soa<T>((T *)c->u, bcur, fr, c->g, (const T *)c->dt, c->st);
             if (e == hipSuccess)
                 e = launch_cfl_soa<T>((const T *)c->u, (const T *)bcur, c->g, c->avg, c->cmax, c->st);
+            if (e == hipSuccess)
+                e = launch_monitors_soa<T>((const T *)c->u, (const T *)bcur, c->g, c->mon_tile, c->mon_nblk, nullptr,
+                                           c->st);
+            if (e == hipSuccess)
+                e = launch_monitor_reduce(c->mon_tile, c->mon_nblk, nullptr, c->mon_hist, c->iter, kMonCap, c->st);
             return e;
         }
-        if (s == 11)
-            return launch_fluidx_soa<T>((T *)c->u, bcur, fr, c->g, (const T *)c->dt, c->st, (T *)c->cfl_part,
-                                        c->avg ? 2 : 1);
+        if (s == 11) {
+            hipError_t e = launch_fluidx_soa<T>((T *)c->u, bcur, fr, c->g, (const T *)c->dt, c->st, (T *)c->cfl_part,
+                                                c->avg ? 2 : 1, c->mon_tile);
+            if (e == hipSuccess)
+                e = launch_monitor_reduce(c->mon_tile, c->nparts, nullptr, c->mon_hist, c->iter, kMonCap, c->st);
+            return e;
+        }
         return launch_fluidx_soa<T>((T *)c->u, bcur, fr, c->g, (const T *)c->dt, c->st);
     }
     // x frame, and y/z frames of small boxes or generic pencil lengths: LDS
@@ -214,7 +228,9 @@ int mhd_ctx_create(mhd_ctx **out, int nx, int ny, int nz, int precision, int avg
               {(void **)&c->cmax, 64},
               {&c->dt, 64},                 {&c->t, 64},               {(void **)&c->t64, 64},
               {&c->dts, (size_t)kDtCap * P}, {(void **)&c->iter, 64},  {(void **)&c->mon_part, 6 * 8 * 1024},
-              {(void **)&c->mon_out, 64}};
+              {(void **)&c->mon_out, 64},
+              {(void **)&c->mon_tile, 6 * 8 * (size_t)(c->nparts > c->mon_nblk ? c->nparts : c->mon_nblk)},
+              {(void **)&c->mon_hist, 6 * 8 * (size_t)kMonCap}};
     for (auto &a : al) {
         e = hipMalloc(a.p, a.n);
         if (e != hipSuccess) {
@@ -243,7 +259,8 @@ int mhd_ctx_destroy(mhd_ctx *c)
     if (!c) return MHD_OK;
     if (c->st) (void)hipStreamSynchronize(c->st);
     drop_graph(c);
-    void *ps[] = {c->u, c->b, c->b2, c->fa, c->cfl_part, c->cmax, c->dt, c->t, c->t64, c->dts, c->iter, c->mon_part, c->mon_out};
+    void *ps[] = {c->u,    c->b,  c->b2,   c->fa,       c->cfl_part, c->cmax,    c->dt,      c->t,
+                  c->t64,  c->dts, c->iter, c->mon_part, c->mon_out,  c->mon_tile, c->mon_hist};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -418,6 +435,23 @@ int mhd_ctx_monitors(mhd_ctx *c, double *out6)
     if (e == hipSuccess) e = hipMemcpyAsync(out6, c->mon_out, 6 * sizeof(double), hipMemcpyDeviceToHost, c->st);
     if (e == hipSuccess) e = hipStreamSynchronize(c->st);
     return e == hipSuccess ? MHD_OK : hip_fail(e, "mhd_ctx_monitors");
+}
+
+int mhd_ctx_monitor_history(mhd_ctx *c, double *h_out, int max, int *count)
+{
+    if (!c || !count) return fail(MHD_ERR_ARG, "mhd_ctx_monitor_history: null argument");
+    int it = 0;
+    hipError_t e = hipStreamSynchronize(c->st);
+    if (e == hipSuccess) e = hipMemcpy(&it, c->iter, sizeof(int), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "mhd_ctx_monitor_history");
+    *count = it;
+    int n = it < max ? it : max;
+    if (n > kMonCap) n = kMonCap;
+    if (n > 0 && h_out) {
+        e = hipMemcpy(h_out, c->mon_hist, (size_t)n * 6 * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(e, "mhd_ctx_monitor_history copy");
+    }
+    return MHD_OK;
 }
 
 int mhd_ctx_device_state(mhd_ctx *c, void **d_u, void **d_b)

@@ -46,11 +46,10 @@ hipError_t launch_rcp_scan(unsigned long long *maxbits, hipStream_t st);
 template <typename T>
 hipError_t launch_transpose(T *uo, T *bo, const T *u, const T *b, int which, int nx, int ny, int nz, hipStream_t st);
 
-// deterministic final reduction of per-block monitor partials (6 per block)
-hipError_t launch_monitor_final(const double *part, int nblk, double *out, hipStream_t st);
-// FP64 monitors: out[6] = mass, energy, momentum x, y, z, max |div B|
-template <typename T>
-hipError_t launch_monitors(const T *u, const T *b, int nx, int ny, int nz, double *part, int nblk, double *out,
-                           hipStream_t st);
+// deterministic reduction of np monitor partial records (6 doubles each:
+// mass, energy, momentum x, y, z, max |div B|) into out[6] (when not null)
+// and, when hist is not null, into hist[6*(*iter-1)..] if that is below cap
+hipError_t launch_monitor_reduce(const double *part, int np, double *out, double *hist, const int *iter, int cap,
+                                 hipStream_t st);
 
 }  // namespace mhd

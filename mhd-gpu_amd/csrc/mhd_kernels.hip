@@ -539,51 +539,44 @@ __global__ void __launch_bounds__(256) transpose_kernel(T *__restrict__ out, con
 }
 
 // ---------------------------------------------------------------------------
-// FP64 monitors: per-block partial sums (mass, energy, momentum x/y/z) and
-// max |div B| (left-face B), reduced in a fixed order -> deterministic.
+// FP64 monitors: the deterministic reduction of np partial records (mass,
+// energy, momentum x, y, z summed; max |div B| max-ed; 6 doubles each).
+// Thread t takes records t, t+nt, ... in order, then a fixed tree over the
+// threads, so the result does not depend on timing.  out[6] when given;
+// with hist, also hist[6*(*iter - 1) ..], the record of the step just taken
+// (its dt kernel advanced *iter), when that index is below cap.
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ void __launch_bounds__(256) monitor_partial_kernel(const T *__restrict__ u, const T *__restrict__ b, int nx,
-                                                              int ny, int nz, double *part)
+__global__ void __launch_bounds__(1024) monitor_reduce_kernel(const double *__restrict__ part, int np, double *out,
+                                                              double *hist, const int *iter, int cap)
 {
-    const long ncell = (long)nx * ny * nz;
-    double a[6] = {0, 0, 0, 0, 0, 0};
-    for (long s = (long)blockIdx.x * blockDim.x + threadIdx.x; s < ncell; s += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(s % nx);
-        const long r = s / nx;
-        const int j = (int)(r % ny), k = (int)(r / ny);
-        const long sip = s - i + (i + 1 == nx ? 0 : i + 1);
-        const long sjp = ((long)k * ny + (j + 1 == ny ? 0 : j + 1)) * nx + i;
-        const long skp = ((long)(k + 1 == nz ? 0 : k + 1) * ny + j) * nx + i;
-        a[0] += (double)u[5 * s];
-        a[1] += (double)u[5 * s + 4];
-        a[2] += (double)u[5 * s + 1];
-        a[3] += (double)u[5 * s + 2];
-        a[4] += (double)u[5 * s + 3];
-        const double d = ((double)b[3 * sip] - (double)b[3 * s]) + ((double)b[3 * sjp + 1] - (double)b[3 * s + 1]) +
-                         ((double)b[3 * skp + 2] - (double)b[3 * s + 2]);
-        a[5] = fmax(a[5], fabs(d));
+    __shared__ double red[6][1024];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double a[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    // unrolled so that the loads of several records are in flight together;
+    // the additions keep their record order
+#pragma unroll 8
+    for (int r = tid; r < np; r += nt) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) a[q] = a[q] + part[6 * (long)r + q];
+        a[5] = fmax(a[5], part[6 * (long)r + 5]);
     }
-    __shared__ double red[6][256];
-    for (int q = 0; q < 6; q++) red[q][threadIdx.x] = a[q];
+#pragma unroll
+    for (int q = 0; q < 6; q++) red[q][tid] = a[q];
     __syncthreads();
-    for (int st = 128; st >= 1; st >>= 1) {
-        if ((int)threadIdx.x < st)
-            for (int q = 0; q < 6; q++)
-                red[q][threadIdx.x] = q == 5 ? fmax(red[q][threadIdx.x], red[q][threadIdx.x + st])
-                                             : red[q][threadIdx.x] + red[q][threadIdx.x + st];
+    for (int st = nt / 2; st >= 1; st >>= 1) {
+        if (tid < st) {
+#pragma unroll
+            for (int q = 0; q < 5; q++) red[q][tid] = red[q][tid] + red[q][tid + st];
+            red[5][tid] = fmax(red[5][tid], red[5][tid + st]);
+        }
         __syncthreads();
     }
-    if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
-}
-
-__global__ void monitor_final_kernel(const double *part, int nblk, double *out)
-{
-    if (threadIdx.x < 6) {
-        const int q = threadIdx.x;
-        double a = 0;
-        for (int i = 0; i < nblk; i++) a = q == 5 ? fmax(a, part[i * 6 + q]) : a + part[i * 6 + q];
-        out[q] = a;
+    if (tid < 6) {
+        if (out) out[tid] = red[tid][0];
+        if (hist) {
+            const int idx = *iter - 1;
+            if (idx >= 0 && idx < cap) hist[6 * (long)idx + tid] = red[tid][0];
+        }
     }
 }
 
@@ -807,18 +800,11 @@ hipError_t launch_transpose(T *uo, T *bo, const T *u, const T *b, int which, int
     return hipGetLastError();
 }
 
-hipError_t launch_monitor_final(const double *part, int nblk, double *out, hipStream_t st)
+hipError_t launch_monitor_reduce(const double *part, int np, double *out, double *hist, const int *iter, int cap,
+                                 hipStream_t st)
 {
-    hipLaunchKernelGGL(monitor_final_kernel, dim3(1), dim3(64), 0, st, part, nblk, out);
+    hipLaunchKernelGGL(monitor_reduce_kernel, dim3(1), dim3(1024), 0, st, part, np, out, hist, iter, cap);
     return hipGetLastError();
-}
-
-template <typename T>
-hipError_t launch_monitors(const T *u, const T *b, int nx, int ny, int nz, double *part, int nblk, double *out,
-                           hipStream_t st)
-{
-    hipLaunchKernelGGL(monitor_partial_kernel<T>, dim3(nblk), dim3(256), 0, st, u, b, nx, ny, nz, part);
-    return launch_monitor_final(part, nblk, out, st);
 }
 
 #define MHD_INST(T)                                                                                                  \
@@ -831,7 +817,6 @@ hipError_t launch_monitors(const T *u, const T *b, int nx, int ny, int nz, doubl
     template hipError_t launch_cfl_value<T>(unsigned long long *, T *, const T *, const T *, int, int, int, int,     \
                                             hipStream_t);                                                           \
     template hipError_t launch_transpose<T>(T *, T *, const T *, const T *, int, int, int, int, hipStream_t);       \
-    template hipError_t launch_monitors<T>(const T *, const T *, int, int, int, double *, int, double *, hipStream_t); \
     template hipError_t prepare_sweeps<T>(int, int, int);
 MHD_INST(float)
 MHD_INST(double)

@@ -48,10 +48,11 @@ bool stream_fused(const StreamBox &sb);
 
 // cflm (x frame only): 1 / 2 also writes the block maxima of the new state's
 // CFL speed (face / cell-averaged B) to cfl_part, one per tile (the x
-// frame's ntj * nk tiles)
+// frame's ntj * nk tiles), and, with mon_part, the tiles' FP64 monitor
+// partials (6 per tile: mass, energy, momentum x, y, z, max |div B|)
 template <typename T>
 hipError_t launch_fluidx_soa(T *u, const T *b, int fr, const SoaGrid &g, const T *dt, hipStream_t st,
-                             T *cfl_part = nullptr, int cflm = 0);
+                             T *cfl_part = nullptr, int cflm = 0, double *mon_part = nullptr);
 // tiles of the x-frame fluidx (entries of its cfl_part)
 template <typename T> inline int fluidx_tiles_x(const SoaGrid &g)
 {

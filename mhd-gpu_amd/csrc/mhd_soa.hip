@@ -43,11 +43,11 @@ __global__ void __launch_bounds__(64 * W)
 template <typename T, int K, int W, bool VEC, int CFLM>
 __global__ void __launch_bounds__(64 * W, K <= 4 ? 4 : 1)
     fluidx_x_kernel(T *__restrict__ u, const T *__restrict__ b, SoaBox bx, const T *__restrict__ dtp, int ntj,
-                    T *__restrict__ cfl_part)
+                    T *__restrict__ cfl_part, double *__restrict__ mon_part)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     fluidx_x_tile<T, K, W, VEC, CFLM>(u, b, bx, dtp, xcd_tile(blockIdx.x, gridDim.x), ntj,
-                                      reinterpret_cast<T *>(smem_raw), cfl_part);
+                                      reinterpret_cast<T *>(smem_raw), cfl_part, nullptr, mon_part);
 }
 
 template <typename T, int K, int W, bool VEC>
@@ -149,17 +149,7 @@ __global__ void __launch_bounds__(256) monitor_soa_kernel(const T *__restrict__ 
             a[5] = fmax(a[5], fabs(d));
         }
     }
-    __shared__ double red[6][256];
-    for (int q = 0; q < 6; q++) red[q][threadIdx.x] = a[q];
-    __syncthreads();
-    for (int st = 128; st >= 1; st >>= 1) {
-        if ((int)threadIdx.x < st)
-            for (int q = 0; q < 6; q++)
-                red[q][threadIdx.x] = q == 5 ? fmax(red[q][threadIdx.x], red[q][threadIdx.x + st])
-                                             : red[q][threadIdx.x] + red[q][threadIdx.x + st];
-        __syncthreads();
-    }
-    if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+    monitor_block_store(a, part + (long)blockIdx.x * 6);
 }
 
 // AoS (logical cell order, C per cell) <-> SoA planes on the padded grid
@@ -259,7 +249,7 @@ template <typename T, int K, int FR> static hipError_t soa_attr()
 
 template <typename T, int K, int FR>
 static hipError_t fluidx_soa_k(T *u, const T *b, const SoaBox &bx, const T *dt, hipStream_t st, T *cfl_part,
-                               int cflm)
+                               int cflm, double *mon_part)
 {
     constexpr int W = soa_w<T>();
     hipError_t e = soa_attr<T, K, FR>();
@@ -272,7 +262,7 @@ static hipError_t fluidx_soa_k(T *u, const T *b, const SoaBox &bx, const T *dt, 
         constexpr bool kvec = K % (16 / sizeof(T)) == 0;
         const size_t xl = sizeof(T) * fluidx_x_words<W>(bx.f.ni, cflm);
 #define MHD_FX(V, C)                                                                                                 \
-    hipLaunchKernelGGL((fluidx_x_kernel<T, K, W, V, C>), grid, block, xl, st, u, b, bx, dt, ntj, cfl_part)
+    hipLaunchKernelGGL((fluidx_x_kernel<T, K, W, V, C>), grid, block, xl, st, u, b, bx, dt, ntj, cfl_part, mon_part)
         if (kvec && bx.vec_ok) {
             if (cflm == 1) MHD_FX(kvec, 1);
             else if (cflm == 2) MHD_FX(kvec, 2);
@@ -338,16 +328,16 @@ static hipError_t advect_soa_k(const T *u, const T *bin, T *bout, const SoaBox &
 
 template <typename T>
 hipError_t launch_fluidx_soa(T *u, const T *b, int fr, const SoaGrid &g, const T *dt, hipStream_t st, T *cfl_part,
-                             int cflm)
+                             int cflm, double *mon_part)
 {
     if (cflm && fr != 0) return hipErrorInvalidValue;
     const SoaBox bx = make_soa_box<T>(fr, g);
     const int K = pick_k(bx.f.ni);
     // any other pencil length: the generic kernels (no fused CFL)
     if (!K) return cflm ? hipErrorInvalidValue : launch_fluidx_gen<T>(u, b, bx.f, bx.N, 1, fr, dt, st);
-#define SF0(KK) fluidx_soa_k<T, KK, 0>(u, b, bx, dt, st, cfl_part, cflm)
-#define SF1(KK) fluidx_soa_k<T, KK, 1>(u, b, bx, dt, st, nullptr, 0)
-#define SF2(KK) fluidx_soa_k<T, KK, 2>(u, b, bx, dt, st, nullptr, 0)
+#define SF0(KK) fluidx_soa_k<T, KK, 0>(u, b, bx, dt, st, cfl_part, cflm, mon_part)
+#define SF1(KK) fluidx_soa_k<T, KK, 1>(u, b, bx, dt, st, nullptr, 0, nullptr)
+#define SF2(KK) fluidx_soa_k<T, KK, 2>(u, b, bx, dt, st, nullptr, 0, nullptr)
     if (fr == 0) { SOA_KSWITCH(K, SF0) }
     if (fr == 1) { SOA_KSWITCH(K, SF1) }
     SOA_KSWITCH(K, SF2)
@@ -501,7 +491,8 @@ hipError_t launch_monitors_soa(const T *u, const T *b, const SoaGrid &g, double 
                                hipStream_t st)
 {
     hipLaunchKernelGGL(monitor_soa_kernel<T>, dim3(nblk), dim3(256), 0, st, u, b, g, part);
-    return launch_monitor_final(part, nblk, out, st);
+    if (!out) return hipGetLastError();  // the partials only
+    return launch_monitor_reduce(part, nblk, out, nullptr, nullptr, 0, st);
 }
 
 template <typename T> hipError_t launch_aos_soa(T *soa, T *aos, int ncomp, const SoaGrid &g, int to_soa, hipStream_t st)
@@ -516,7 +507,7 @@ template <typename T> hipError_t launch_aos_soa(T *soa, T *aos, int ncomp, const
 #define SOA_INST(T)                                                                                                  \
     template SoaBox make_soa_box<T>(int, const SoaGrid &);                                                          \
     template hipError_t launch_fluidx_soa<T>(T *, const T *, int, const SoaGrid &, const T *, hipStream_t, T *,   \
-                                             int);                                                                  \
+                                             int, double *);                                                        \
     template hipError_t launch_advect_soa<T>(const T *, const T *, T *, int, const SoaGrid &, const T *,           \
                                              hipStream_t);                                                          \
     template hipError_t prepare_soa<T>(int, int, int);                                                              \

@@ -894,7 +894,8 @@ template <int W> __host__ __device__ constexpr int fluidx_x_words(int n, int cfl
 template <typename T, int K, int W, bool VEC, int CFLM>
 __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__restrict__ b, const SoaBox &bx,
                                               const T *__restrict__ dtp, int tile, int ntj, T *su,
-                                              T *__restrict__ cfl_part, unsigned long long *cmax = nullptr)
+                                              T *__restrict__ cfl_part, unsigned long long *cmax = nullptr,
+                                              double *__restrict__ mon_part = nullptr)
 {
     const Frame &f = bx.f;
     const TileCtx t = tile_ctx<W>(bx, tile, ntj);
@@ -964,13 +965,17 @@ __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__rest
         // (fluidx does not write b); shadow lanes repeat lane l % L
         __syncthreads();
         T c = (T)0;
+        // with mon_part: the FP64 monitors of the new state (mass, energy,
+        // momentum x, y, z, max |div B| on the left faces) of the tile's cells
+        const bool mon = mon_part != nullptr;
+        double ms[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         if (w < ns) {
             T b1[K], b2[K], b3[K];
             load(b + p, b1);
             load(b + N + p, b2);
             load(b + 2 * N + p, b3);
             T b1p[K], b2p[K], b3p[K];
-            if constexpr (CFLM == 2) {
+            if (CFLM == 2 || mon) {
                 load(b + N + pj, b2p);
                 load(b + 2 * N + pk, b3p);
                 shift_next<T, K>(b1, b1p, src_next);
@@ -992,7 +997,21 @@ __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__rest
                 const T sp = cfl_speed<T>(um[0][q], um[1][q], um[2][q], um[3][q], um[4][q], B);
                 c = cfl_max(c, sp);  // max_2num (calcfl.c:55)
             }
+            if (mon && l < L) {
+#pragma unroll
+                for (int q = 0; q < K; q++) {
+                    ms[0] += (double)um[0][q];
+                    ms[1] += (double)um[4][q];
+                    ms[2] += (double)um[1][q];
+                    ms[3] += (double)um[2][q];
+                    ms[4] += (double)um[3][q];
+                    const double d = ((double)b1p[q] - (double)b1[q]) + ((double)b2p[q] - (double)b2[q]) +
+                                     ((double)b3p[q] - (double)b3[q]);
+                    ms[5] = fmax(ms[5], fabs(d));
+                }
+            }
         }
+        if (mon) monitor_block_store(ms, mon_part + (long)tile * 6);
         c = wave_cfl_max(c);
         if (lane_id() == 0) cfl_red_slot<T>()[w] = c;
         __syncthreads();

@@ -65,6 +65,7 @@ SIGNATURES = {
     "mhd_ctx_dts": (_I, [_P, _DP, _I, _IP]),
     "mhd_ctx_time": (_I, [_P, _DP, _DP]),
     "mhd_ctx_monitors": (_I, [_P, _DP]),
+    "mhd_ctx_monitor_history": (_I, [_P, _DP, _I, _IP]),
     "mhd_ctx_device_state": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
     "mhd_ctx_layout": (_I, [_P, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long),
                             ctypes.POINTER(ctypes.c_long)]),
@@ -299,6 +300,14 @@ class Simulation:
         out = np.zeros(6, np.float64)
         check(lib().mhd_ctx_monitors(self.ctx, out.ctypes.data_as(_DP)))
         return out
+
+    def monitor_history(self, maxn=1 << 16):
+        """FP64 monitors recorded after every step: rows of (mass, energy,
+        momentum x, y, z, max |div B|)."""
+        cnt = ctypes.c_int()
+        out = np.zeros((maxn, 6), np.float64)
+        check(lib().mhd_ctx_monitor_history(self.ctx, out.ctypes.data_as(_DP), maxn, ctypes.byref(cnt)))
+        return out[:min(cnt.value, maxn)]
 
     def profile_step(self, reps=3):
         out = (ctypes.c_float * 13)()
