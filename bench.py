@@ -67,42 +67,66 @@ def valu_roofline(fl_ms, n):
             "floor_ms": floor_ms, "avg_launch_ms": fl_ms, "frac": floor_ms / fl_ms}
 
 
-def cpu_baseline(u0, b0, n, seconds_hint=30.0):
-    """The reference's own C path (oracle/_ref, serial, as the reference runs)
-    on one step of the same workload, or the oracle restatement if the
-    reference build is absent."""
+def host_info():
+    """The host the CPU baselines ran on: logical CPUs this process may use,
+    the lscpu model and socket / core / thread counts, OMP_NUM_THREADS."""
+    import subprocess
+    info = {"nproc": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "NUMA node(s)": "numa_nodes"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                v = v.strip()
+                info[keys[k.strip()]] = int(v) if v.isdigit() else v
+        if "sockets" in info and "cores_per_socket" in info:
+            info["physical_cores"] = info["sockets"] * info["cores_per_socket"]
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_baselines(u0, b0, n):
+    """The CPU path beside the GPU one (reported baselines, not the target):
+    * the reference's own C path (oracle/_ref, compiled from its sources;
+      serial, as the reference runs) on 1 step of the n^3 workload (about
+      14 s at 256^3) and on the 32^3 linear Alfven workload in full (200
+      steps); where oracle/_ref is absent, the oracle restatement instead;
+    * the oracle restatement (the reference C path's evaluation order) with
+      its pencil loops spread over OpenMP threads, as the reference's Fortran
+      OpenMP driver spreads them (OpenMP/mhd.f90), on 3 steps of n^3."""
     import oracle_lib
-    u, b = u0.copy(), b0.copy()
+    from mhdgpu.ics import alfven_linear
     if oracle_lib.reference_available():
-        ref = oracle_lib.Reference()
-        kind, cores = "reference", 1
-        t0 = time.perf_counter()
-        ref.run(u, b, n, n, n, 1)
-        dt = time.perf_counter() - t0
+        serial, kind, cores = oracle_lib.Reference(), "reference", 1
     else:
-        orc = oracle_lib.Oracle()
-        kind, cores = "port", os.cpu_count() or 1
-        t0 = time.perf_counter()
-        orc.run(u, b, n, n, n, 1)
-        dt = time.perf_counter() - t0
-    return {"value": n ** 3 / dt, "unit": "zone-updates/s", "cores": cores, "kind": kind,
-            "sample": f"1 step of the {n}^3 workload (same initial state), {dt:.1f} s wall"}
-
-
-def cpu_baseline_omp(u0, b0, n):
-    """The oracle restatement (the reference's C evaluation order) with its
-    pencil loops spread over OpenMP threads, as the reference's Fortran OpenMP
-    driver spreads them (OpenMP/mhd.f90), on one step of the same workload:
-    the multi-core CPU figure beside the serial reference one."""
-    import oracle_lib
+        serial, kind, cores = oracle_lib.Oracle(), "port", oracle_lib.Oracle().threads()
+    out = {}
+    u, b = u0.copy(), b0.copy()
+    t0 = time.perf_counter()
+    serial.run(u, b, n, n, n, 1)
+    dt = time.perf_counter() - t0
+    out["cpu_baseline"] = {"value": n ** 3 / dt, "unit": "zone-updates/s", "cores": cores, "kind": kind,
+                           "sample": f"1 step of the {n}^3 workload (same initial state), {dt:.1f} s wall"}
+    u, b = alfven_linear(32)
+    t0 = time.perf_counter()
+    serial.run(u, b, 32, 32, 32, 200)
+    dt = time.perf_counter() - t0
+    out["cpu_baseline_config1"] = {"value": 32 ** 3 * 200 / dt, "unit": "zone-updates/s", "cores": cores,
+                                   "kind": kind, "sample": f"32^3 linear Alfven wave, 200 steps in full, {dt:.1f} s wall"}
     orc = oracle_lib.Oracle()
     u, b = u0.copy(), b0.copy()
     t0 = time.perf_counter()
-    orc.run(u, b, n, n, n, 1)
+    orc.run(u, b, n, n, n, 3)
     dt = time.perf_counter() - t0
-    return {"value": n ** 3 / dt, "unit": "zone-updates/s", "cores": orc.threads(), "kind": "port",
-            "sample": f"1 step of the {n}^3 workload (same initial state), {dt:.2f} s wall, "
-                      f"OpenMP over pencils"}
+    out["cpu_baseline_omp"] = {"value": 3 * n ** 3 / dt, "unit": "zone-updates/s", "cores": orc.threads(),
+                               "kind": "port", "sample": f"3 steps of the {n}^3 workload (same initial state), "
+                                                         f"{dt:.2f} s wall, OpenMP over pencils"}
+    out["cpu_host"] = host_info()
+    return out
 
 
 def bench_batched(mhdgpu, steps=100, nbox=64, seed=3):
@@ -208,15 +232,19 @@ def bench_served(mhdgpu, n_req=64):
 VARS = ["rho", "mx", "my", "mz", "E", "bx", "by", "bz"]
 
 
-def precision_report(mhdgpu, n, steps, ic, seed):
+def precision_report(mhdgpu, n, steps, ic, seed, state=None):
     """FP32 against FP64 on the device from the same initial state: relative
     L2 error per variable after `steps` steps, FP64 drift of the total mass
-    and energy, max |div B| (left faces), and the dt of both runs."""
+    and energy, max |div B| (left faces), and the dt of both runs.  state:
+    the initial (u, b) when the caller has generated it already."""
     from mhdgpu.ics import random_phase, turbulence
-    log(f"precision: generating {ic} {n}^3 (seed {seed})")
-    if ic == "turbulence":
+    if state is not None:
+        u0, b0 = state
+    elif ic == "turbulence":
+        log(f"precision: generating {ic} {n}^3 (seed {seed})")
         u0, b0 = turbulence(n, seed=seed, progress=log)
     else:
+        log(f"precision: generating {ic} {n}^3 (seed {seed})")
         u0, b0 = random_phase(n, seed=seed)
     runs = {}
     for prec in (8, 4):
@@ -264,6 +292,61 @@ def precision_report(mhdgpu, n, steps, ic, seed):
     return out
 
 
+def steplevel_report(mhdgpu, n, steps, warmup):
+    """The reference-shaped path: the loop of HIP/cu/cuda_main.cu:89-121 driven
+    through the step-level entry points on AoS device arrays (mhd_cfl, then
+    dt = 0.9*cfl clamped as Heterogeneous_MHD/C/mhd.c:63-65 on the host, then
+    fluidx / advectbyzx with the permuted dims and the four transposes, which
+    write ut, bt and copy back).  Every entry point is synchronous, like the
+    reference's wrappers; per-op times are host wall times around each call."""
+    from mhdgpu.ics import random_phase
+    u0, b0 = random_phase(n, seed=2)
+    du, db = mhdgpu.DeviceArray.from_host(u0), mhdgpu.DeviceArray.from_host(b0)
+    dut, dbt = mhdgpu.DeviceArray(u0.nbytes), mhdgpu.DeviceArray(b0.nbytes)
+    dcfl, ddt = mhdgpu.DeviceArray(4), mhdgpu.DeviceArray(4)
+    acc = {}
+
+    def op(name, fn, *a):
+        t0 = time.perf_counter()
+        fn(*a)
+        acc[name] = acc.get(name, 0.0) + time.perf_counter() - t0
+
+    M = mhdgpu
+    X = (n, n, n)
+    seq = [("fluidx", X), ("advect", X), ("t12", X), ("fluidx", X), ("advect", X), ("t13", X), ("fluidx", X),
+           ("advect", X), ("advect", X), ("fluidx", X), ("t13", X), ("advect", X), ("fluidx", X), ("t12", X),
+           ("advect", X), ("fluidx", X)]
+
+    def step():
+        t0 = time.perf_counter()
+        c = M.cfl(du, db, n, n, n, dcfl)
+        dt = np.float32(0.9 * float(np.float32(c)))
+        ddt.copy_from(np.array([dt], np.float32))
+        acc["cfl+dt"] = acc.get("cfl+dt", 0.0) + time.perf_counter() - t0
+        for name, d in seq:
+            if name == "fluidx":
+                op(name, M.fluidx, du, db, *d, ddt)
+            elif name == "advect":
+                op(name, M.advectbyzx, du, db, *d, ddt)
+            elif name == "t12":
+                op(name, M.transpose12, dut, dbt, du, db, *d)
+            else:
+                op(name, M.transpose13, dut, dbt, du, db, *d)
+
+    for _ in range(warmup):
+        step()
+    acc.clear()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    wall = time.perf_counter() - t0
+    counts = {"fluidx": 6, "advect": 6, "t12": 2, "t13": 2, "cfl+dt": 1}
+    return {"mode": "steplevel", "grid": [n, n, n], "steps": steps, "ms_per_step": wall * 1e3 / steps,
+            "zone_updates_per_s": n ** 3 * steps / wall,
+            "ms_per_call": {k: v * 1e3 / (steps * counts[k]) for k, v in acc.items()},
+            "note": "synchronous calls; host wall time per call includes the launch and the sync"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -273,10 +356,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--requests", type=int, default=64)
-    ap.add_argument("--mode", choices=["plain", "copy", "precision"], default="plain",
+    ap.add_argument("--mode", choices=["plain", "copy", "precision", "steplevel"], default="plain",
                     help="copy: time the plain copy kernel on 1 GiB (counter calibration); "
-                         "precision: FP32 vs FP64 report on --ic at --n for --steps")
+                         "precision: FP32 vs FP64 report on --ic at --n for --steps; "
+                         "steplevel: the reference's loop through the step-level AoS entry points")
     ap.add_argument("--ic", choices=["random_phase", "turbulence"], default="random_phase")
     ap.add_argument("--seed", type=int, default=2)
     args = ap.parse_args()
@@ -298,6 +383,10 @@ def main():
         wall = time.perf_counter() - t0
         print(json.dumps({"mode": "copy", "bytes_copied": nbytes, "launches": args.steps,
                           "GBs_read_plus_write": 2 * nbytes * args.steps / wall / 1e9}), flush=True)
+        return
+
+    if args.mode == "steplevel":
+        print(json.dumps(steplevel_report(mhdgpu, args.n, args.steps, args.warmup)), flush=True)
         return
 
     if args.mode == "precision":
@@ -324,17 +413,24 @@ def main():
     # per-kernel times with HIP events on the context's stream
     prof = sim.profile_step(reps=5)
     sweep_ms = prof[1:]
-    # the last sweep (fluidx x) also forms the next step's CFL: the plain
-    # fluidx average is over the other five
+    # the dominant kernel is the fluidx of the y and z frames (fluidx_soa_kernel,
+    # four launches per step); the x-frame fluidx is reported beside it, and
+    # the last sweep (fluidx x) also forms the next step's CFL and monitors
+    fl_yz = [sweep_ms[i] for i, s in enumerate(SWEEPS) if s in ("fluidx_y", "fluidx_z")]
     fl = [sweep_ms[i] for i, s in enumerate(SWEEPS) if s.startswith("fluidx") and i != 11]
     ad = [sweep_ms[i] for i, s in enumerate(SWEEPS) if s.startswith("advect")]
+    fl_yz_ms = float(np.mean(fl_yz))
     fl_ms = float(np.mean(fl))
     ad_ms = float(np.mean(ad))
     cells = n ** 3
-    achieved = FLUIDX_BYTES_PER_CELL * cells / (fl_ms * 1e-3) / 1e9
+    achieved = FLUIDX_BYTES_PER_CELL * cells / (fl_yz_ms * 1e-3) / 1e9
     m = sim.monitors()
     t, t64 = sim.time()
     finite = bool(np.isfinite(m).all())
+    hist = sim.monitor_history()
+    mass_drift = float((hist[-1, 0] - hist[0, 0]) / hist[0, 0]) if len(hist) else None
+    energy_drift = float((hist[-1, 1] - hist[0, 1]) / hist[0, 1]) if len(hist) else None
+    max_divb = float(hist[-1, 5]) if len(hist) else None
     sim.close()
 
     out = {
@@ -353,14 +449,21 @@ def main():
         "config": {"workload": f"{n}^3 periodic FP32 box, Alfven background + random-phase velocity (rms Mach 0.1) "
                                f"+ curl-A field (rms 0.1 B0), plain method, {args.steps} timed steps",
                    "grid": [n, n, n], "method": "plain", "precision": "fp32"},
-        "roofline": {"kernel": "fluidx", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic("fluidx", n),
-                     "bytes_per_launch": FLUIDX_BYTES_PER_CELL * cells, "avg_launch_ms": fl_ms},
+        "roofline": {"kernel": "fluidx_soa_kernel (fluidx, y and z frames)", "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": pmc_traffic("fluidx", n), "bytes_per_launch": FLUIDX_BYTES_PER_CELL * cells,
+                     "avg_launch_ms": fl_yz_ms,
+                     "timing": "HIP events around each launch on the context's stream (mhd_ctx_profile_step, "
+                               "5 steps)",
+                     "trace_avg_launch_ms": profile_value(f"step_{n}cube_kernel_trace.txt", "fluidx_yz_trace_avg_ms")},
         "valu_roofline": valu_roofline(fl_ms, n),
+        "monitors_fp64_per_step": {"recorded": int(len(hist)), "mass_drift_over_timed_steps": mass_drift,
+                                   "energy_drift_over_timed_steps": energy_drift, "max_divB_last": max_divb},
         "step_roofline": {"bytes_per_zone_update": STEP_BYTES_PER_CELL,
                           "achieved_GBs": zus * STEP_BYTES_PER_CELL / 1e9,
                           "frac": zus * STEP_BYTES_PER_CELL / 1e9 / HBM_PEAK_GBS},
-        "kernel_ms": {"dt": float(prof[0]), "fluidx_avg": fl_ms, "fluidx_x_with_cfl": float(sweep_ms[11]),
+        "kernel_ms": {"dt": float(prof[0]), "fluidx_yz_avg": fl_yz_ms, "fluidx_avg_5": fl_ms,
+                      "fluidx_x": float(sweep_ms[0]), "fluidx_x_with_cfl_and_monitors": float(sweep_ms[11]),
                       "advect_avg": ad_ms,
                       "sweeps": {f"{i:02d}_{s}": float(v) for i, (s, v) in enumerate(zip(SWEEPS, sweep_ms))}},
         "monitors_fp64": {"mass": m[0], "energy": m[1], "max_divB": m[5], "t": t, "t64": t64, "finite": finite},
@@ -382,10 +485,18 @@ def main():
         out["batched"] = bench_batched(mhdgpu)
     if not args.no_served:
         out["served"] = bench_served(mhdgpu, args.requests)
+    if not args.no_precision:
+        log("bench: precision, FP32 against FP64, 100 steps")
+        pr = precision_report(mhdgpu, n, 100, "random_phase", 2, state=(u0, b0))
+        worst = max(pr["rel_l2_fp32_vs_fp64"], key=pr["rel_l2_fp32_vs_fp64"].get)
+        out["precision_fp32_vs_fp64"] = {
+            "workload": f"the same {n}^3 state, 100 steps, FP32 and FP64 on the device",
+            "rel_l2_all": pr["rel_l2_all"], "rel_l2_worst": {worst: pr["rel_l2_fp32_vs_fp64"][worst]},
+            "fp32": {k: pr["fp32"][k] for k in ("mass_drift", "energy_drift", "max_divB_end", "ms_per_step")},
+            "fp64": {k: pr["fp64"][k] for k in ("mass_drift", "energy_drift", "max_divB_end", "ms_per_step")}}
     if not args.no_cpu:
-        log("bench: CPU baseline")
-        out["cpu_baseline"] = cpu_baseline(u0, b0, n)
-        out["cpu_baseline_omp"] = cpu_baseline_omp(u0, b0, n)
+        log("bench: CPU baselines")
+        out.update(cpu_baselines(u0, b0, n))
     print(json.dumps(out), flush=True)
 
 

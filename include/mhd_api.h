@@ -98,6 +98,18 @@ int mhd_transpose12(float *ut, float *bt, float *u, float *b, const int *nx, con
  * permutation is exact for any shape. */
 int mhd_transpose13(float *ut, float *bt, float *u, float *b, const int *nx, const int *ny, const int *nz);
 
+/* replaces cuda_transpose12_ (HIP/cu/cuda_transpose12.cu:69-127), the
+ * Fortran-callable form: trailing underscore, every argument by reference,
+ * HOST arrays; `call mhd_transpose12(ut, bt, u, b, nx, ny, nz, dt)` from
+ * Fortran links to it.  u, b are copied to the device, transposed there and
+ * the result copied into ut, bt (u, b unchanged; the caller may pass the
+ * same arrays for ut and u).  dt is unused, as in the reference.  The
+ * transpose13 form is its sibling (no Fortran form in the reference). */
+int mhd_transpose12_(float *ut, float *bt, const float *u, const float *b, const int *nx, const int *ny,
+                     const int *nz, const float *dt);
+int mhd_transpose13_(float *ut, float *bt, const float *u, const float *b, const int *nx, const int *ny,
+                     const int *nz, const float *dt);
+
 /* replaces cuda_cfl (HIP/cu/cuda_cfl.cu:14) and calcfl (Heterogeneous_MHD/C/calcfl.c:13):
  * cfl = 1/max over cells of max|v| + sqrt(|2B^2 + gamma p|/rho), the C
  * path's face-value B^2 (avg_b = 0) or the cell-averaged B^2 of
@@ -128,7 +140,10 @@ int mhd_write_result(const char *path_u, const char *path_b, const float *u, con
  * loop iterations; nsteps <= 0 runs while t < tf like the reference
  * (tf must then be finite).  dt = 0.9*cfl clamped to (tf-t)/2 as in
  * Heterogeneous_MHD/C/mhd.c:63-65; pass tf = INFINITY for no clamp.  When
- * h_dts is not NULL it receives the dt of each step (capacity max_dts). */
+ * h_dts is not NULL it receives the dt of each step (capacity max_dts).
+ * A NaN or infinite dt (the state held NaN, a non-positive density or no
+ * signal speed; the steps follow the reference's arithmetic regardless)
+ * returns MHD_ERR_NONFINITE after the state and the dts are written back. */
 int mhd_main(float *h_u, float *h_b, const int *nx, const int *ny, const int *nz, int nsteps, float tf,
              float *h_dts, int max_dts, int *steps_done);
 

@@ -301,6 +301,41 @@ int mhd_transpose13(float *ut, float *bt, float *u, float *b, const int *nx, con
     return transpose_common(13, ut, bt, u, b, nx, ny, nz);
 }
 
+// host arrays in, the transposed frame out into h_ut, h_bt (h_u, h_b are
+// read before anything is written, so a caller may pass the same arrays)
+static int transpose_host(int which, float *h_ut, float *h_bt, const float *h_u, const float *h_b, const int *nx,
+                          const int *ny, const int *nz)
+{
+    if (!h_ut || !h_bt || !h_u || !h_b || !nx || !ny || !nz) return fail(MHD_ERR_ARG, "mhd_transpose_: null argument");
+    if (*nx < 1 || *ny < 1 || *nz < 1) return fail(MHD_ERR_SHAPE, "mhd_transpose_: bad grid");
+    const size_t nc = (size_t)*nx * *ny * *nz, bu = nc * 5 * sizeof(float), bb = nc * 3 * sizeof(float);
+    float *d = nullptr;
+    hipError_t e = hipMalloc(&d, 2 * (bu + bb));
+    if (e != hipSuccess) return fail(MHD_ERR_NOMEM, "mhd_transpose_: hipMalloc: %s", hipGetErrorString(e));
+    float *du = d, *db = du + 5 * nc, *dut = db + 3 * nc, *dbt = dut + 5 * nc;
+    e = hipMemcpy(du, h_u, bu, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(db, h_b, bb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_transpose<float>(dut, dbt, du, db, which, *nx, *ny, *nz, 0);
+    if (e == hipSuccess) e = hipMemcpy(h_ut, dut, bu, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(h_bt, dbt, bb, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    return e == hipSuccess ? MHD_OK : hip_fail(e, "mhd_transpose_");
+}
+
+int mhd_transpose12_(float *h_ut, float *h_bt, const float *h_u, const float *h_b, const int *nx, const int *ny,
+                     const int *nz, const float *h_dt)
+{
+    (void)h_dt;  // unused, as in the reference
+    return transpose_host(12, h_ut, h_bt, h_u, h_b, nx, ny, nz);
+}
+
+int mhd_transpose13_(float *h_ut, float *h_bt, const float *h_u, const float *h_b, const int *nx, const int *ny,
+                     const int *nz, const float *h_dt)
+{
+    (void)h_dt;
+    return transpose_host(13, h_ut, h_bt, h_u, h_b, nx, ny, nz);
+}
+
 int mhd_cfl(const float *u, const float *b, const int *nx, const int *ny, const int *nz, int avg_b, float *d_cfl,
             float *h_cfl)
 {
@@ -401,14 +436,29 @@ int mhd_main(float *h_u, float *h_b, const int *nx, const int *ny, const int *nz
     }
     if (!rc) rc = mhd_ctx_sync(ctx);
     if (!rc) rc = mhd_ctx_get_state(ctx, h_u, h_b);
-    if (!rc && h_dts && max_dts > 0) {
-        std::vector<double> d(max_dts);
+    // the dt sequence: into h_dts, and checked (a NaN or infinite dt means the
+    // state holds NaN, a non-positive density or an all-zero speed)
+    int bad_step = -1;
+    float bad_dt = 0.0f;
+    if (!rc && done > 0) {
+        std::vector<double> d(done);
         int cnt = 0;
-        rc = mhd_ctx_dts(ctx, d.data(), max_dts, &cnt);
-        for (int i = 0; i < cnt && i < max_dts; i++) h_dts[i] = (float)d[i];
+        rc = mhd_ctx_dts(ctx, d.data(), done, &cnt);
+        for (int i = 0; !rc && i < cnt && i < done; i++) {
+            if (h_dts && i < max_dts) h_dts[i] = (float)d[i];
+            if (bad_step < 0 && !isfinite(d[i])) {
+                bad_step = i;
+                bad_dt = (float)d[i];
+            }
+        }
     }
     if (steps_done) *steps_done = done;
     const int rc2 = mhd_ctx_destroy(ctx);
+    if (!rc && !rc2 && bad_step >= 0)
+        return fail(MHD_ERR_NONFINITE,
+                    "mhd_main: step %d has dt = %g (the state holds NaN, a non-positive density or no signal speed); "
+                    "the state and the dts were written back",
+                    bad_step, (double)bad_dt);
     return rc ? rc : rc2;
 }
 

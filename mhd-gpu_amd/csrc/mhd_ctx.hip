@@ -41,8 +41,8 @@ struct mhd_ctx {
     double *mon_tile;          // per-tile monitor partials of the step's last sweep (6 per x-frame tile)
     double *mon_hist;          // FP64 monitors after each step (6 per step, kMonCap steps)
     double tf;
-    hipGraph_t graph;
-    hipGraphExec_t exec;
+    hipGraph_t graph, graph_n;     // one step; kGraphSteps steps
+    hipGraphExec_t exec, exec_n;
     bool captured;
     int adv_form;              // y/z advectbyzx: 0 automatic, 1 tiles, 2 two-pass stream, 3 fused stream
     hipEvent_t ev0, ev1;
@@ -50,6 +50,9 @@ struct mhd_ctx {
 
 static const int kDtCap = 1 << 20;
 static const int kMonCap = 1 << 16;
+// steps per replay of the long graph: one graph launch costs ~9 us of idle
+// GPU between steps (measured on MI355X), paid once per kGraphSteps steps
+static const int kGraphSteps = 8;
 
 // Padding of the SoA rows and planes (in cells).  A plane of 256x256 floats
 // is 256 KiB: z-sweep tiles read rows one plane apart, and at a power-of-two
@@ -157,12 +160,15 @@ static hipError_t enqueue_step_any(mhd_ctx *c)
     return c->prec == 4 ? enqueue_step<float>(c) : enqueue_step<double>(c);
 }
 
-static int ensure_graph(mhd_ctx *c)
+static void drop_graph(mhd_ctx *c);
+
+// capture `steps` steps of the loop into one executable graph
+static int capture_steps(mhd_ctx *c, int steps, hipGraph_t *gout, hipGraphExec_t *xout)
 {
-    if (c->captured) return MHD_OK;
     hipError_t e = hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
-    hipError_t el = enqueue_step_any(c);
+    hipError_t el = hipSuccess;
+    for (int s = 0; s < steps && el == hipSuccess; s++) el = enqueue_step_any(c);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(c->st, &g);
     if (el != hipSuccess) {
@@ -176,18 +182,42 @@ static int ensure_graph(mhd_ctx *c)
         (void)hipGraphDestroy(g);
         return hip_fail(e, "hipGraphInstantiate");
     }
-    c->graph = g;
-    c->exec = x;
+    *gout = g;
+    *xout = x;
+    return MHD_OK;
+}
+
+static int ensure_graph(mhd_ctx *c)
+{
+    if (c->captured) return MHD_OK;
+    int rc = capture_steps(c, 1, &c->graph, &c->exec);
+    if (!rc) rc = capture_steps(c, kGraphSteps, &c->graph_n, &c->exec_n);
+    if (rc) {
+        drop_graph(c);
+        return rc;
+    }
     c->captured = true;
     return MHD_OK;
+}
+
+// enqueue nsteps steps: replays of the long graph, then single steps
+static hipError_t launch_steps(mhd_ctx *c, int nsteps)
+{
+    hipError_t e = hipSuccess;
+    int s = 0;
+    for (; s + kGraphSteps <= nsteps && e == hipSuccess; s += kGraphSteps) e = hipGraphLaunch(c->exec_n, c->st);
+    for (; s < nsteps && e == hipSuccess; s++) e = hipGraphLaunch(c->exec, c->st);
+    return e;
 }
 
 static void drop_graph(mhd_ctx *c)
 {
     if (c->exec) (void)hipGraphExecDestroy(c->exec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
-    c->exec = nullptr;
-    c->graph = nullptr;
+    if (c->exec_n) (void)hipGraphExecDestroy(c->exec_n);
+    if (c->graph_n) (void)hipGraphDestroy(c->graph_n);
+    c->exec = c->exec_n = nullptr;
+    c->graph = c->graph_n = nullptr;
     c->captured = false;
 }
 
@@ -229,7 +259,8 @@ int mhd_ctx_create(mhd_ctx **out, int nx, int ny, int nz, int precision, int avg
               {&c->dt, 64},                 {&c->t, 64},               {(void **)&c->t64, 64},
               {&c->dts, (size_t)kDtCap * P}, {(void **)&c->iter, 64},  {(void **)&c->mon_part, 6 * 8 * 1024},
               {(void **)&c->mon_out, 64},
-              {(void **)&c->mon_tile, 6 * 8 * (size_t)(c->nparts > c->mon_nblk ? c->nparts : c->mon_nblk)},
+              // the tile records and, after them, the reduction's first stage
+              {(void **)&c->mon_tile, 6 * 8 * (size_t)((c->nparts > c->mon_nblk ? c->nparts : c->mon_nblk) * 2)},
               {(void **)&c->mon_hist, 6 * 8 * (size_t)kMonCap}};
     for (auto &a : al) {
         e = hipMalloc(a.p, a.n);
@@ -355,11 +386,8 @@ int mhd_ctx_step(mhd_ctx *c, int nsteps)
     if (!c || nsteps < 0) return fail(MHD_ERR_ARG, "mhd_ctx_step: bad argument");
     int rc = ensure_graph(c);
     if (rc) return rc;
-    for (int s = 0; s < nsteps; s++) {
-        hipError_t e = hipGraphLaunch(c->exec, c->st);
-        if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
-    }
-    return MHD_OK;
+    const hipError_t e = launch_steps(c, nsteps);
+    return e == hipSuccess ? MHD_OK : hip_fail(e, "hipGraphLaunch");
 }
 
 int mhd_ctx_step_timed(mhd_ctx *c, int nsteps, float *ms)
@@ -368,7 +396,7 @@ int mhd_ctx_step_timed(mhd_ctx *c, int nsteps, float *ms)
     int rc = ensure_graph(c);
     if (rc) return rc;
     hipError_t e = hipEventRecord(c->ev0, c->st);
-    for (int s = 0; s < nsteps && e == hipSuccess; s++) e = hipGraphLaunch(c->exec, c->st);
+    if (e == hipSuccess) e = launch_steps(c, nsteps);
     if (e == hipSuccess) e = hipEventRecord(c->ev1, c->st);
     if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
     if (e == hipSuccess) e = hipEventElapsedTime(ms, c->ev0, c->ev1);

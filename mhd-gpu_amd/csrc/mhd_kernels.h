@@ -48,7 +48,9 @@ hipError_t launch_transpose(T *uo, T *bo, const T *u, const T *b, int which, int
 
 // deterministic reduction of np monitor partial records (6 doubles each:
 // mass, energy, momentum x, y, z, max |div B|) into out[6] (when not null)
-// and, when hist is not null, into hist[6*(*iter-1)..] if that is below cap
+// and, when hist is not null, into hist[6*(*iter-1)..] if that is below cap.
+// Above 2048 records part must have room for ceil(np/256) more records
+// after the np inputs (a first reduction stage writes them there).
 hipError_t launch_monitor_reduce(const double *part, int np, double *out, double *hist, const int *iter, int cap,
                                  hipStream_t st);
 

@@ -549,33 +549,45 @@ __global__ void __launch_bounds__(256) transpose_kernel(T *__restrict__ out, con
 __global__ void __launch_bounds__(1024) monitor_reduce_kernel(const double *__restrict__ part, int np, double *out,
                                                               double *hist, const int *iter, int cap)
 {
-    __shared__ double red[6][1024];
     const int tid = threadIdx.x, nt = blockDim.x;
+    const double2 *p2 = reinterpret_cast<const double2 *>(part);  // a record is three double2
     double a[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    // unrolled so that the loads of several records are in flight together;
-    // the additions keep their record order
-#pragma unroll 8
-    for (int r = tid; r < np; r += nt) {
+    auto add = [&](const double2 (&v)[3]) {
+        a[0] = a[0] + v[0].x;
+        a[1] = a[1] + v[0].y;
+        a[2] = a[2] + v[1].x;
+        a[3] = a[3] + v[1].y;
+        a[4] = a[4] + v[2].x;
+        a[5] = fmax(a[5], v[2].y);
+    };
+    // four records per round: their loads are issued before any addition,
+    // the additions keep the record order
+    int r = tid;
+    for (; r + 3 * nt < np; r += 4 * nt) {
+        double2 v[4][3];
 #pragma unroll
-        for (int q = 0; q < 5; q++) a[q] = a[q] + part[6 * (long)r + q];
-        a[5] = fmax(a[5], part[6 * (long)r + 5]);
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int h = 0; h < 3; h++) v[k][h] = p2[3 * (long)(r + k * nt) + h];
+#pragma unroll
+        for (int k = 0; k < 4; k++) add(v[k]);
     }
+    for (; r < np; r += nt) {
+        double2 v[3];
 #pragma unroll
-    for (int q = 0; q < 6; q++) red[q][tid] = a[q];
+        for (int h = 0; h < 3; h++) v[h] = p2[3 * (long)r + h];
+        add(v);
+    }
+    // the threads in a fixed order: a butterfly in each wave, then the waves
+    // in index order (monitor_block_store)
+    __shared__ double res[6];
+    monitor_block_store(a, res);
     __syncthreads();
-    for (int st = nt / 2; st >= 1; st >>= 1) {
-        if (tid < st) {
-#pragma unroll
-            for (int q = 0; q < 5; q++) red[q][tid] = red[q][tid] + red[q][tid + st];
-            red[5][tid] = fmax(red[5][tid], red[5][tid + st]);
-        }
-        __syncthreads();
-    }
     if (tid < 6) {
-        if (out) out[tid] = red[tid][0];
+        if (out) out[tid] = res[tid];
         if (hist) {
             const int idx = *iter - 1;
-            if (idx >= 0 && idx < cap) hist[6 * (long)idx + tid] = red[tid][0];
+            if (idx >= 0 && idx < cap) hist[6 * (long)idx + tid] = res[tid];
         }
     }
 }
@@ -800,9 +812,39 @@ hipError_t launch_transpose(T *uo, T *bo, const T *u, const T *b, int which, int
     return hipGetLastError();
 }
 
+// first stage of a large reduction: block g reduces records [256g, 256g+256)
+// (one per thread, then monitor_block_store's fixed order) into stage[g]
+__global__ void __launch_bounds__(256) monitor_stage_kernel(const double *__restrict__ part, int np,
+                                                            double *__restrict__ stage)
+{
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    double a[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (r < np) {
+        const double2 *p2 = reinterpret_cast<const double2 *>(part) + 3 * (long)r;
+        const double2 v0 = p2[0], v1 = p2[1], v2 = p2[2];
+        a[0] = v0.x;
+        a[1] = v0.y;
+        a[2] = v1.x;
+        a[3] = v1.y;
+        a[4] = v2.x;
+        a[5] = v2.y;
+    }
+    monitor_block_store(a, stage + 6 * (long)blockIdx.x);
+}
+
 hipError_t launch_monitor_reduce(const double *part, int np, double *out, double *hist, const int *iter, int cap,
                                  hipStream_t st)
 {
+    if (np > 2048) {
+        // one workgroup alone reads 16 K records at about 40 GB/s: a first
+        // stage over 256-record groups, on as many workgroups, writes its
+        // records after the np inputs (the caller's buffer holds them)
+        const int ng = (np + 255) / 256;
+        double *stage = const_cast<double *>(part) + 6 * (long)np;
+        hipLaunchKernelGGL(monitor_stage_kernel, dim3(ng), dim3(256), 0, st, part, np, stage);
+        hipLaunchKernelGGL(monitor_reduce_kernel, dim3(1), dim3(1024), 0, st, stage, ng, out, hist, iter, cap);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(monitor_reduce_kernel, dim3(1), dim3(1024), 0, st, part, np, out, hist, iter, cap);
     return hipGetLastError();
 }

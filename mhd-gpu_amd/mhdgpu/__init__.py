@@ -49,6 +49,8 @@ SIGNATURES = {
     "mhd_advectbyzx_noalloc": (_I, [_P, _P, _IP, _IP, _IP, _P]),
     "mhd_transpose12": (_I, [_P, _P, _P, _P, _IP, _IP, _IP]),
     "mhd_transpose13": (_I, [_P, _P, _P, _P, _IP, _IP, _IP]),
+    "mhd_transpose12_": (_I, [_P, _P, _P, _P, _IP, _IP, _IP, _FP]),
+    "mhd_transpose13_": (_I, [_P, _P, _P, _P, _IP, _IP, _IP, _FP]),
     "mhd_cfl": (_I, [_P, _P, _IP, _IP, _IP, _I, _P, _FP]),
     "mhd_init": (_I, [ctypes.c_char_p, _I, _P, _P, _IP, _IP, _IP]),
     "mhd_write_result": (_I, [ctypes.c_char_p, ctypes.c_char_p, _P, _P, _IP, _IP, _IP]),
@@ -190,6 +192,18 @@ def transpose12(ut, bt, u, b, nx, ny, nz):
 def transpose13(ut, bt, u, b, nx, ny, nz):
     """(i,j,k)->(k,j,i); afterwards u, b hold the (nz,ny,nx) frame (transpose13.c:9)."""
     check(lib().mhd_transpose13(ut.ptr, bt.ptr, u.ptr, b.ptr, _iref(nx), _iref(ny), _iref(nz)))
+
+
+def transpose_host(which, u, b, nx, ny, nz):
+    """The Fortran-callable host-array transposes (cuda_transpose12_,
+    HIP/cu/cuda_transpose12.cu:69): returns (ut, bt); u, b are unchanged."""
+    u = np.ascontiguousarray(u, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    ut, bt = np.empty_like(u), np.empty_like(b)
+    fn = lib().mhd_transpose12_ if which == 12 else lib().mhd_transpose13_
+    check(fn(ut.ctypes.data, bt.ctypes.data, u.ctypes.data, b.ctypes.data, _iref(nx), _iref(ny), _iref(nz),
+             ctypes.byref(ctypes.c_float(0.0))))
+    return ut, bt
 
 
 def cfl(u, b, nx, ny, nz, d_cfl, avg_b=False):

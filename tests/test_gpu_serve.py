@@ -78,3 +78,43 @@ def test_server_budget_queues_and_refuses(M):
 
 def test_fifo_baseline_results_equal_alone(M):
     run_server(M, mode=1)
+
+
+def test_served_poisson_mix_matches_oracle(M, oracle):
+    """The served workload, reduced: seeded Poisson arrivals of requests with
+    N drawn from {32, 48, 64, 96}, random-phase initial states with their own
+    seeds and 10-40 steps each, continuous batching; every result equals the
+    oracle's run of that request (the sign of exact zeros aside)."""
+    from conftest import same
+    from mhdgpu.ics import random_phase
+    rng = np.random.default_rng(55)
+    nreq = 12
+    sizes = rng.choice(np.array([32, 48, 64, 96]), nreq).tolist()
+    steps = rng.integers(10, 41, nreq).tolist()
+    gaps = rng.exponential(1.0 / 200.0, nreq)  # mean 200 requests/s
+    states = [random_phase(n, seed=5000 + i) for i, n in enumerate(sizes)]
+    sv = M.Server(mode=0)
+    outs = {}
+    t0 = time.perf_counter()
+    arrive = np.cumsum(gaps)
+    for i, n in enumerate(sizes):
+        while time.perf_counter() - t0 < arrive[i]:
+            time.sleep(0.0005)
+        rid, out = sv.submit((n, n, n), steps[i], *states[i])
+        outs[rid] = (i, out)
+    done = {}
+    while len(done) < nreq and time.perf_counter() - t0 < 120:
+        for r in sv.poll():
+            done[r["id"]] = r
+        time.sleep(0.002)
+    stats = sv.stats()
+    sv.close()
+    assert len(done) == nreq
+    assert stats["zone_updates"] == sum(n ** 3 * k for n, k in zip(sizes, steps))
+    for rid, (i, (ou, ob)) in outs.items():
+        assert done[rid]["rc"] == 0
+        n = sizes[i]
+        print("served request", i, n, steps[i], flush=True)
+        uo, bo = (a.copy() for a in states[i])
+        oracle.run(uo, bo, n, n, n, steps[i])
+        assert same(ou, uo) and same(ob, bo), i

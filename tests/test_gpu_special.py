@@ -108,3 +108,26 @@ def test_batch_box_with_nan_cfl(M, oracle):
         assert same(bt.dts(i), do), i
         assert same(ug, uo) and same(bg, bo), i
     bt.close()
+
+
+def test_main_reports_nonfinite_dt(M, oracle):
+    # mhd_main runs the reference's arithmetic on a state with a NaN energy,
+    # writes the state and the dts back, and returns MHD_ERR_NONFINITE
+    import ctypes
+    n = 10
+    u0, b0 = special_state("nan_energy", n, n, n, seed=5)
+    u, b = u0.copy(), b0.copy()
+    dts = np.zeros(3, F32)
+    done = ctypes.c_int()
+    ni = ctypes.c_int(n)
+    rc = M.lib().mhd_main(u.ctypes.data, b.ctypes.data, ctypes.byref(ni), ctypes.byref(ni), ctypes.byref(ni), 3,
+                          ctypes.c_float(np.inf), dts.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), 3,
+                          ctypes.byref(done))
+    assert rc == -8 and done.value == 3
+    assert "dt" in M.lib().mhd_last_error().decode()
+    uo, bo = u0.copy(), b0.copy()
+    do = oracle.run(uo, bo, n, n, n, 3)
+    assert same(dts, do) and same(u, uo) and same(b, bo)
+    # a finite run returns MHD_OK through the same path
+    u, b = special_state("neg_density", n, n, n, seed=5)
+    assert np.all(np.isfinite(M.main(u, b, n, n, n, 3)))
