@@ -111,9 +111,12 @@ int mhd_transpose13_(float *ut, float *bt, const float *u, const float *b, const
                      const int *nz, const float *dt);
 
 /* replaces cuda_cfl (HIP/cu/cuda_cfl.cu:14) and calcfl (Heterogeneous_MHD/C/calcfl.c:13):
- * cfl = 1/max over cells of max|v| + sqrt(|2B^2 + gamma p|/rho), the C
- * path's face-value B^2 (avg_b = 0) or the cell-averaged B^2 of
- * OpenMP/mhd.f90:112-120 (avg_b = 1).  Written to *d_cfl (device) and, when
+ * cfl = 1/max over cells of max|v| + sqrt(|2B^2 + gamma p|/rho), with the
+ * C path's face-value B^2 (avg_b = 0, calcfl.c:45-56), or (avg_b = 1) the
+ * same C-path arithmetic with B^2 of the cell-averaged field
+ * ((b(i)+b(i+1))/2 per axis, the averages calcfl.c:38-43 forms and then
+ * leaves unused).  The running max keeps calcfl's max_2num semantics: a NaN
+ * speed resets it (calcfl.c:55).  Written to *d_cfl (device) and, when
  * h_cfl is not NULL, to *h_cfl (host).  The caller forms dt = 0.9*cfl as
  * Heterogeneous_MHD/C/mhd.c:63 does (the GPU reference instead used 1/c as
  * dt on the device, HIP/cu/cuda_cfl.cu:41-43). */

@@ -242,8 +242,8 @@ __device__ __forceinline__ void mhdflux_cell_bulk(const float *u, const float *b
                min(__builtin_amdgcn_frexp_expf(bm), __builtin_amdgcn_frexp_expf(w)));
 }
 
-// Per-cell fast speed of calcfl.c:45-54 (face B at the cell, or the cell
-// average of OpenMP/mhd.f90:112-120 when the caller passes averaged b).
+// Per-cell fast speed of calcfl.c:45-54 (B^2 of the face values at the cell,
+// or of the cell averages calcfl.c:38-43 forms when the caller passes them).
 template <typename T>
 __device__ __forceinline__ T cfl_speed(T u1, T u2, T u3, T u4, T u5, T B)
 {

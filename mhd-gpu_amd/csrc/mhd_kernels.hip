@@ -328,9 +328,11 @@ __global__ void __launch_bounds__(64 * W) advect_kernel(const T *__restrict__ u,
 }
 
 // ---------------------------------------------------------------------------
-// cfl: Heterogeneous_MHD/C/calcfl.c:13-65 -> max over cells, as IEEE bits in
-// *cmax (all speeds are >= 0, so the unsigned order is the float order).
-// AVG selects the cell-averaged B^2 (OpenMP/mhd.f90:112-120).
+// cfl: Heterogeneous_MHD/C/calcfl.c:13-65 -> max over cells, as the
+// NaN-sticky unsigned bits of speed_bits in *cmax (finite speeds are >= 0, so
+// their unsigned order is the float order; any NaN outranks them).  AVG
+// selects B^2 of the cell-averaged field that calcfl.c:38-43 forms, with the
+// C path's arithmetic.
 // ---------------------------------------------------------------------------
 template <typename T> __device__ __forceinline__ unsigned long long tobits(T x)
 {

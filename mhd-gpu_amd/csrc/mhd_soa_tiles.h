@@ -58,7 +58,7 @@ __device__ __forceinline__ T cfl_speed_at(const T *__restrict__ u, const T *__re
     const long N = g.N;
     T B;
     if constexpr (AVG) {
-        // cell-averaged field (OpenMP/mhd.f90:112-120)
+        // cell-averaged field (calcfl.c:38-43), C-path arithmetic
         const T bxa = (T)((double)(b1 + b[sip]) / 2.0);
         const T bya = (T)((double)(b2 + b[N + sjp]) / 2.0);
         const T bza = (T)((double)(b3 + b[2 * N + skp]) / 2.0);
