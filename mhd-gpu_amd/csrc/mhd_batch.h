@@ -23,6 +23,7 @@ struct BoxDesc {
     SoaGrid g;
     SoaBox fr[3];        // the three sweep frames
     int K[3];            // cells per lane of each frame
+    int KF[3], PF[3];    // the fluidx sweeps' cells per lane and pencils per wave (short pencils share a wave)
     int ntj[3];          // tiles along j of each frame
     StreamBox sb[3];     // streamed advectbyzx of frames 1 and 2
     int nwa[3], nwb[3];  // its waves (pass A, pass B)
@@ -36,11 +37,13 @@ struct CflItem {
     int box, e0, ne;  // units e0 .. e0+ne-1 of the box (see cfl_range)
 };
 
-// boxes with the same cells-per-lane K in every frame (schedule 3): their
+// boxes with the same cells-per-lane K in every frame, and the same K and
+// pencils per wave of their fluidx sweeps (schedule 3): their
 // items, CFL items and active slots are contiguous ranges of the engine's
 // lists, and they run exact-K kernels
 struct BoxGroup {
     int K[3];
+    int KF[3], PF[3];
     int item_off[3], item_n[3];
     int cfl_off, cfl_n;
     int act_off, act_n;

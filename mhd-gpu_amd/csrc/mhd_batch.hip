@@ -199,8 +199,9 @@ __global__ void __launch_bounds__(256) batch_stream_b_kernel(const BoxDesc *__re
 // (schedule 3): the x-frame group kernel's body, each block adding its max
 // CFL speed to the box's accumulator, which the next step's dt kernel reads.
 // CFLM 1 / 2: face / cell-averaged B.
-template <int K, int CFLM>
-__global__ void __launch_bounds__(64 * kBW)
+// P pencils per wave (short pencils, kBW/P waves per block)
+template <int K, int CFLM, int P = 1>
+__global__ void __launch_bounds__(64 * kBW / P)
     batch_fluidx_x_cfl_kernel(const BoxDesc *__restrict__ boxes, const WorkItem *__restrict__ items,
                               const float *__restrict__ dt, int flip, unsigned long long *__restrict__ cmax)
 {
@@ -210,27 +211,61 @@ __global__ void __launch_bounds__(64 * kBW)
     const BoxDesc &bd = boxes[wi.box];
     const float *bcur = flip ? bd.b2 : bd.b;
     if (K % 4 == 0 && bd.fr[0].vec_ok)
-        fluidx_x_tile<float, K, kBW, K % 4 == 0, CFLM>(bd.u, bcur, bd.fr[0], dt + wi.box, wi.tile, bd.ntj[0], smem,
-                                                        nullptr, cmax + wi.box);
+        fluidx_x_tile<float, K, kBW, K % 4 == 0, CFLM, P>(bd.u, bcur, bd.fr[0], dt + wi.box, wi.tile, bd.ntj[0],
+                                                           smem, nullptr, cmax + wi.box);
     else
-        fluidx_x_tile<float, K, kBW, false, CFLM>(bd.u, bcur, bd.fr[0], dt + wi.box, wi.tile, bd.ntj[0], smem,
-                                                   nullptr, cmax + wi.box);
+        fluidx_x_tile<float, K, kBW, false, CFLM, P>(bd.u, bcur, bd.fr[0], dt + wi.box, wi.tile, bd.ntj[0], smem,
+                                                      nullptr, cmax + wi.box);
 }
 
-static const void *fluidx_x_cfl_kernel(int K, int cflm)
+static const void *fluidx_x_cfl_kernel(int K, int P, int cflm)
 {
-#define MHD_FC(KK) return cflm == 2 ? (const void *)batch_fluidx_x_cfl_kernel<KK, 2> : (const void *)batch_fluidx_x_cfl_kernel<KK, 1>
+#define MHD_FC(KK, PP)                                                                                               \
+    return cflm == 2 ? (const void *)batch_fluidx_x_cfl_kernel<KK, 2, PP> : (const void *)batch_fluidx_x_cfl_kernel<KK, 1, PP>
+    if (P == 2) {
+        if (K == 1) MHD_FC(1, 2);
+        if (K == 3) MHD_FC(3, 2);
+        return nullptr;
+    }
+    if (P == 4) {
+        if (K == 1) MHD_FC(1, 4);
+        if (K == 3) MHD_FC(3, 4);
+        return nullptr;
+    }
     switch (K) {
-    case 1: MHD_FC(1);
-    case 2: MHD_FC(2);
-    case 3: MHD_FC(3);
-    case 4: MHD_FC(4);
-    case 5: MHD_FC(5);
-    case 6: MHD_FC(6);
-    case 8: MHD_FC(8);
+    case 1: MHD_FC(1, 1);
+    case 2: MHD_FC(2, 1);
+    case 3: MHD_FC(3, 1);
+    case 4: MHD_FC(4, 1);
+    case 5: MHD_FC(5, 1);
+    case 6: MHD_FC(6, 1);
+    case 8: MHD_FC(8, 1);
     default: return nullptr;
     }
 #undef MHD_FC
+}
+
+// the step's first sweep (fluidx, x frame) with P > 1 pencils per wave
+template <int K, int P>
+__global__ void __launch_bounds__(64 * kBW / P)
+    batch_fluidx_x_p_kernel(const BoxDesc *__restrict__ boxes, const WorkItem *__restrict__ items,
+                            const float *__restrict__ dt, int flip)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float *smem = reinterpret_cast<float *>(smem_raw);
+    const WorkItem wi = items[xcd_tile(blockIdx.x, gridDim.x)];
+    const BoxDesc &bd = boxes[wi.box];
+    fluidx_x_tile<float, K, kBW, false, 0, P>(bd.u, flip ? bd.b2 : bd.b, bd.fr[0], dt + wi.box, wi.tile, bd.ntj[0],
+                                              smem, nullptr);
+}
+
+static const void *fluidx_x_p_kernel(int K, int P)
+{
+    if (P == 2) return K == 1 ? (const void *)batch_fluidx_x_p_kernel<1, 2>
+                              : K == 3 ? (const void *)batch_fluidx_x_p_kernel<3, 2> : nullptr;
+    if (P == 4) return K == 1 ? (const void *)batch_fluidx_x_p_kernel<1, 4>
+                              : K == 3 ? (const void *)batch_fluidx_x_p_kernel<3, 4> : nullptr;
+    return nullptr;
 }
 
 // x-frame advectbyzx in tiles of two planes (schedule 3): one block per item
@@ -268,8 +303,8 @@ static const void *advect_x2_kernel(int K)
 
 // y/z fluidx in tiles of two vectors (schedule 3): one block of 8 waves per
 // item (box, tile of 8 pencils), exact K
-template <int FR, int K>
-__global__ void __launch_bounds__(64 * 2 * kBW)
+template <int FR, int K, int P = 1>
+__global__ void __launch_bounds__(64 * 2 * kBW / P)
     batch_fluidx_yz_kernel(const BoxDesc *__restrict__ boxes, const WorkItem *__restrict__ items,
                            const float *__restrict__ dt, int flip)
 {
@@ -278,16 +313,32 @@ __global__ void __launch_bounds__(64 * 2 * kBW)
     const BoxDesc &bd = boxes[wi.box];
     const int ntj = (bd.fr[FR].f.nj + 2 * kBW - 1) / (2 * kBW);
     float *smem = reinterpret_cast<float *>(smem_raw);
-    // a pencil that fills the wave: its length as a constant (uniform branch)
-    if (bd.fr[FR].f.ni == 64 * K)
+    if constexpr (P > 1) {
+        // several short pencils per wave
+        fluidx_soa_tile<float, K, 2 * kBW, FR, 0, 0, P>(bd.u, flip ? bd.b2 : bd.b, bd.fr[FR], dt + wi.box, wi.tile,
+                                                        ntj, smem);
+    } else if (bd.fr[FR].f.ni == 64 * K) {
+        // a pencil that fills the wave: its length as a constant (uniform branch)
         fluidx_soa_tile<float, K, 2 * kBW, FR, 0, 64 * K>(bd.u, flip ? bd.b2 : bd.b, bd.fr[FR], dt + wi.box, wi.tile,
                                                           ntj, smem);
-    else
+    } else {
         fluidx_soa_tile<float, K, 2 * kBW, FR>(bd.u, flip ? bd.b2 : bd.b, bd.fr[FR], dt + wi.box, wi.tile, ntj, smem);
+    }
 }
 
-static const void *fluidx_yz_kernel(int fr, int K)
+static const void *fluidx_yz_kernel(int fr, int K, int P)
 {
+    if (P == 2 || P == 4) {
+#define MHD_FYP(F, PP)                                                                                               \
+    if (fr == F && P == PP) return K == 1 ? (const void *)batch_fluidx_yz_kernel<F, 1, PP>                          \
+                                          : K == 3 ? (const void *)batch_fluidx_yz_kernel<F, 3, PP> : nullptr;
+        MHD_FYP(1, 2)
+        MHD_FYP(2, 2)
+        MHD_FYP(1, 4)
+        MHD_FYP(2, 4)
+#undef MHD_FYP
+        return nullptr;
+    }
 #define MHD_FY(F)                                                                                                    \
     if (fr == F) switch (K) {                                                                                        \
         case 1: return (const void *)batch_fluidx_yz_kernel<F, 1>;                                                   \
@@ -601,13 +652,23 @@ int Engine::init()
                 e = hipFuncSetAttribute(group_kernel(fr, K, adv), hipFuncAttributeMaxDynamicSharedMemorySize,
                                         128 * 1024);
             if (fr > 0 && e == hipSuccess)
-                e = hipFuncSetAttribute(fluidx_yz_kernel(fr, K), hipFuncAttributeMaxDynamicSharedMemorySize,
+                e = hipFuncSetAttribute(fluidx_yz_kernel(fr, K, 1), hipFuncAttributeMaxDynamicSharedMemorySize,
                                         160 * 1024);
             if (fr == 0 && e == hipSuccess)
                 e = hipFuncSetAttribute(advect_x2_kernel(K), hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024);
             for (int cflm = 1; fr == 0 && cflm <= 2 && e == hipSuccess; cflm++)
-                e = hipFuncSetAttribute(fluidx_x_cfl_kernel(K, cflm), hipFuncAttributeMaxDynamicSharedMemorySize,
+                e = hipFuncSetAttribute(fluidx_x_cfl_kernel(K, 1, cflm), hipFuncAttributeMaxDynamicSharedMemorySize,
                                         128 * 1024);
+        }
+    // the short-pencil fluidx kernels (n <= 96: a few tens of KB of LDS)
+    for (int P : {2, 4})
+        for (int K : {1, 3}) {
+            for (int fr = 1; fr < 3 && e == hipSuccess; fr++)
+                e = hipFuncSetAttribute(fluidx_yz_kernel(fr, K, P), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        64 * 1024);
+            for (int cflm = 1; cflm <= 2 && e == hipSuccess; cflm++)
+                e = hipFuncSetAttribute(fluidx_x_cfl_kernel(K, P, cflm), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        64 * 1024);
         }
     if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute");
     // side streams (default priority) and the fork/join events of the
@@ -688,6 +749,8 @@ int Engine::claim_slot(int nx, int ny, int nz, hipStream_t st, int *slot)
     for (int fr = 0; fr < 3; fr++) {
         d.fr[fr] = make_soa_box<float>(fr, d.g);
         d.K[fr] = pick_k(d.fr[fr].f.ni);
+        d.PF[fr] = pick_p(d.fr[fr].f.ni);
+        d.KF[fr] = d.PF[fr] > 1 ? d.fr[fr].f.ni * d.PF[fr] / 64 : d.K[fr];
         d.ntj[fr] = (d.fr[fr].f.nj + kBW - 1) / kBW;
         if (fr > 0) {
             d.sb[fr] = make_stream_box(fr, d.g);
@@ -940,17 +1003,23 @@ int Engine::build()
     // the most cells first; the lists below then hold each group contiguously
     groups_.clear();
     if (policy_ == 3) {
+        const BoxGroup G0{};
         std::vector<std::vector<int>> members;
         for (int s : order) {
             size_t g = 0;
             for (; g < groups_.size(); g++)
-                if (groups_[g].K[0] == boxes_[s].K[0] && groups_[g].K[1] == boxes_[s].K[1] &&
-                    groups_[g].K[2] == boxes_[s].K[2])
+                if (memcmp(groups_[g].K, boxes_[s].K, sizeof G0.K) == 0 &&
+                    memcmp(groups_[g].KF, boxes_[s].KF, sizeof G0.KF) == 0 &&
+                    memcmp(groups_[g].PF, boxes_[s].PF, sizeof G0.PF) == 0)
                     break;
             if (g == groups_.size()) {
                 BoxGroup G;
                 memset(&G, 0, sizeof G);
-                for (int fr = 0; fr < 3; fr++) G.K[fr] = boxes_[s].K[fr];
+                for (int fr = 0; fr < 3; fr++) {
+                    G.K[fr] = boxes_[s].K[fr];
+                    G.KF[fr] = boxes_[s].KF[fr];
+                    G.PF[fr] = boxes_[s].PF[fr];
+                }
                 groups_.push_back(G);
                 members.emplace_back();
             }
@@ -973,6 +1042,20 @@ int Engine::build()
             for (int s : m) stream_now_[s] = together || boxes_[s].streamed;
             std::stable_partition(m.begin(), m.end(), [&](int s) { return !stream_now_[s]; });
         }
+        // short pencils keep several per wave only where that pays for the
+        // group's launch as a whole (p_pays over its boxes' pencils); every
+        // box of a group with PF > 1 has the same pencil length
+        for (size_t g = 0; g < groups_.size(); g++)
+            for (int fr = 0; fr < 3; fr++) {
+                BoxGroup &G = groups_[g];
+                if (G.PF[fr] == 1) continue;
+                long pencils = 0;
+                for (int s : members[g]) pencils += (long)boxes_[s].fr[fr].f.nj * boxes_[s].fr[fr].f.nk;
+                if (p_pays(boxes_[members[g][0]].fr[fr].f.ni, pencils) == 1) {
+                    G.PF[fr] = 1;
+                    G.KF[fr] = G.K[fr];
+                }
+            }
         std::vector<int> gi(groups_.size());
         for (size_t g = 0; g < gi.size(); g++) gi[g] = (int)g;
         std::stable_sort(gi.begin(), gi.end(), [&](int a, int b) { return groups_[a].cells > groups_[b].cells; });
@@ -1221,16 +1304,22 @@ int Engine::enqueue_step()
                     e = k ? hipLaunchKernel(k, dim3(G.x2_n), dim3(64 * kBW), args, G.lds_x2, S) : hipErrorInvalidValue;
                 } else if (sw == 11) {
                     // the last sweep (fluidx, x frame) also forms the next step's CFL
-                    const void *k = fluidx_x_cfl_kernel(G.K[0], avg_ ? 2 : 1);
+                    const void *k = fluidx_x_cfl_kernel(G.KF[0], G.PF[0], avg_ ? 2 : 1);
                     const WorkItem *it = d_items_[0] + G.item_off[0];
                     void *args[] = {(void *)&d_boxes_, (void *)&it, (void *)&d_dt_, (void *)&flip, (void *)&d_cmax_};
-                    e = k ? hipLaunchKernel(k, dim3(G.item_n[0]), dim3(64 * kBW), args, G.lds_f[0], S)
+                    e = k ? hipLaunchKernel(k, dim3(G.item_n[0]), dim3(64 * kBW / G.PF[0]), args, G.lds_f[0], S)
                           : hipErrorInvalidValue;
                 } else if (!adv && fr > 0 && G.f8_n[fr] > 0) {
-                    const void *k = fluidx_yz_kernel(fr, G.K[fr]);
+                    const void *k = fluidx_yz_kernel(fr, G.KF[fr], G.PF[fr]);
                     const WorkItem *it = d_f8items_ + G.f8_off[fr];
                     void *args[] = {(void *)&d_boxes_, (void *)&it, (void *)&d_dt_, (void *)&flip};
-                    e = k ? hipLaunchKernel(k, dim3(G.f8_n[fr]), dim3(64 * 2 * kBW), args, G.lds_f8[fr], S)
+                    e = k ? hipLaunchKernel(k, dim3(G.f8_n[fr]), dim3(64 * 2 * kBW / G.PF[fr]), args, G.lds_f8[fr], S)
+                          : hipErrorInvalidValue;
+                } else if (!adv && fr == 0 && G.PF[0] > 1) {
+                    const void *k = fluidx_x_p_kernel(G.KF[0], G.PF[0]);
+                    const WorkItem *it = d_items_[0] + G.item_off[0];
+                    void *args[] = {(void *)&d_boxes_, (void *)&it, (void *)&d_dt_, (void *)&flip};
+                    e = k ? hipLaunchKernel(k, dim3(G.item_n[0]), dim3(64 * kBW / G.PF[0]), args, 0, S)
                           : hipErrorInvalidValue;
                 } else {
                     e = launch_group(fr, G.K[fr], adv, G.item_n[fr], adv ? G.lds_a[fr] : G.lds_f[fr], S, d_boxes_,

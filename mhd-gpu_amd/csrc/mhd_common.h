@@ -331,11 +331,36 @@ __device__ __forceinline__ LanesRagged ragged_lanes(int n, int K, int lc)
     return g;
 }
 
+// Several pencils per wave: 64/G pencils of n = G*K cells, pencil g on the
+// aligned lanes g*G .. g*G+G-1 (lane l holds cells (l % G)*K .. of pencil
+// l / G).  next/prev wrap inside the lane group, and the pencil max is a
+// reduction over the group only.
+template <int G> struct LanesGroup {
+    static_assert(G >= 2 && G <= 32 && (G & (G - 1)) == 0, "a lane group is a power of two below the wave");
+    int next, prev;
+    __device__ __forceinline__ bool valid(int) const { return true; }
+};
+template <int G> __device__ __forceinline__ LanesGroup<G> group_lanes(int l)
+{
+    const int base = l & ~(G - 1), lc = l & (G - 1);
+    return LanesGroup<G>{base + ((lc + 1) & (G - 1)), base + ((lc + G - 1) & (G - 1))};
+}
+
 template <typename T, int K> __device__ __forceinline__ void shift_next(const T (&a)[K], T (&o)[K], const LanesFull &g)
 {
     shift_next<T, K>(a, o, g.next);
 }
 template <typename T, int K> __device__ __forceinline__ void shift_prev(const T (&a)[K], T (&o)[K], const LanesFull &g)
+{
+    shift_prev<T, K>(a, o, g.prev);
+}
+template <typename T, int K, int G>
+__device__ __forceinline__ void shift_next(const T (&a)[K], T (&o)[K], const LanesGroup<G> &g)
+{
+    shift_next<T, K>(a, o, g.next);
+}
+template <typename T, int K, int G>
+__device__ __forceinline__ void shift_prev(const T (&a)[K], T (&o)[K], const LanesGroup<G> &g)
 {
     shift_prev<T, K>(a, o, g.prev);
 }
@@ -368,6 +393,15 @@ template <typename T> __device__ __forceinline__ T wave_max(T x)
 {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) x = fmax(x, shfl_from(x, lane_id() ^ off));
+    return x;
+}
+// the pencil max of each lane geometry: the whole wave, or the lane group
+template <typename T> __device__ __forceinline__ T pencil_max(T x, const LanesFull &) { return wave_max(x); }
+template <typename T> __device__ __forceinline__ T pencil_max(T x, const LanesRagged &) { return wave_max(x); }
+template <typename T, int G> __device__ __forceinline__ T pencil_max(T x, const LanesGroup<G> &)
+{
+#pragma unroll
+    for (int off = G / 2; off >= 1; off >>= 1) x = fmax(x, shfl_from(x, lane_id() ^ off));
     return x;
 }
 

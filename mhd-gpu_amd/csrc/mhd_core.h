@@ -2,8 +2,10 @@
 // a pencil of n cells lives in one wavefront, lane l < L holding cells
 // l*K .. l*K+K-1 in registers; lanes >= L shadow lane l % L.  Neighbours along
 // the pencil come from lane shuffles, the pencil max from a wave reduction.
-// The lane geometry LN is LanesFull (n = L*K) or LanesRagged (the last lane
-// holds fewer than K cells; its other cells stay out of every reduction).
+// The lane geometry LN is LanesFull (n = L*K), LanesRagged (the last lane
+// holds fewer than K cells; its other cells stay out of every reduction) or
+// LanesGroup<G> (64/G pencils of G*K cells share the wave, each on its own
+// aligned group of G lanes; shifts and the max stay inside the group).
 #pragma once
 #include <type_traits>
 
@@ -17,7 +19,7 @@ template <typename T, int K, class LN>
 __device__ __forceinline__ void mhdflux_pencil(const T (&uu)[K][5], const T (&bx)[K][3], T (&v)[K][5], T &c,
                                                const LN &ln)
 {
-    constexpr bool whole = std::is_same<LN, LanesFull>::value;
+    constexpr bool whole = !std::is_same<LN, LanesRagged>::value;
     T cl = (T)0;
     if constexpr (sizeof(T) == 4) {
         // the divisions by u1 through one reciprocal per cell (exact while no
@@ -63,11 +65,11 @@ __device__ __forceinline__ void mhdflux_pencil(const T (&uu)[K][5], const T (&bx
             if (ln.valid(q)) cl = fmax(cl, ci);
         }
     }
-    c = wave_max(cl);
+    c = pencil_max(cl, ln);
     if (c > (T)0) {
         if constexpr (sizeof(T) == 4) {
             // v/c for the 5K fluxes of the lane: c is uniform over the
-            // wave, so its reciprocal is formed once and each quotient takes
+            // pencil's lanes, so its reciprocal is formed once and each quotient takes
             // the remaining five operations of the IEEE division sequence;
             // that is exact while no operand scaling would trigger (Recip).
             // The operand ranges are checked in bulk: c finite with exponent

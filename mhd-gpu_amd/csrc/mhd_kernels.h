@@ -8,6 +8,16 @@ namespace mhd {
 
 // smallest supported cells-per-lane K with 64*K >= n and n % K == 0; 0 if none
 int pick_k(int n);
+// pencils per wave of the FP32 sweeps whose pencils fill a wave several at a
+// time: 2 or 4 when n = (64/P)*K with K = 1 or 3 (n = 16, 32, 48, 96), else 1
+int pick_p(int n);
+// pick_p(n) where several pencils per wave pay for a launch over `pencils`
+// pencils: where they keep pick_k(n) cells per lane, or where the launch
+// still has 4096 waves (four per SIMD); else 1.  Sharing a wave at a larger K
+// lengthens each lane's work, which a launch too small to fill the GPU pays
+// for in latency (48^3 measured 6% slower at 4 pencils per wave, 96^3 7%
+// faster at 2).
+int p_pays(int n, long pencils);
 Frame make_frame(int fr, int nx, int ny, int nz);
 
 // sweeps over an x-major AoS box nx*ny*nz in frame fr (0: x, 1: y, 2: z)

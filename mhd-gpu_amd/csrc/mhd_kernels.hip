@@ -606,6 +606,22 @@ int pick_k(int n)
     return 0;
 }
 
+int pick_p(int n)
+{
+    for (int P : {2, 4}) {
+        const int G = 64 / P;
+        if (n % G == 0 && (n / G == 1 || n / G == 3)) return P;
+    }
+    return 1;
+}
+
+int p_pays(int n, long pencils)
+{
+    const int P = pick_p(n);
+    if (P == 1) return 1;
+    return n * P / 64 == pick_k(n) || pencils / P >= 4096 ? P : 1;
+}
+
 Frame make_frame(int fr, int nx, int ny, int nz)
 {
     Frame f;

@@ -19,14 +19,15 @@
 
 namespace mhd {
 
-// y/z frames (LDS tiles); NF as in fluidx_soa_tile
-template <typename T, int K, int W, int FR, int NF = 0>
-__global__ void __launch_bounds__(64 * W)
+// y/z frames (LDS tiles); NF and P (pencils per wave, W/P waves) as in
+// fluidx_soa_tile
+template <typename T, int K, int W, int FR, int NF = 0, int P = 1>
+__global__ void __launch_bounds__(64 * W / P)
     fluidx_soa_kernel(T *__restrict__ u, const T *__restrict__ b, SoaBox bx, const T *__restrict__ dtp, int ntj)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    fluidx_soa_tile<T, K, W, FR, 0, NF>(u, b, bx, dtp, xcd_tile(blockIdx.x, gridDim.x), ntj,
-                                        reinterpret_cast<T *>(smem_raw));
+    fluidx_soa_tile<T, K, W, FR, 0, NF, P>(u, b, bx, dtp, xcd_tile(blockIdx.x, gridDim.x), ntj,
+                                           reinterpret_cast<T *>(smem_raw));
 }
 
 template <typename T, int K, int W, int FR>
@@ -39,15 +40,15 @@ __global__ void __launch_bounds__(64 * W)
                                  reinterpret_cast<T *>(smem_raw));
 }
 
-// at most 128 VGPRs for K <= 4 (four waves per SIMD)
-template <typename T, int K, int W, bool VEC, int CFLM>
-__global__ void __launch_bounds__(64 * W, K <= 4 ? 4 : 1)
+// at most 128 VGPRs for K <= 4 (four waves per SIMD); P pencils per wave
+template <typename T, int K, int W, bool VEC, int CFLM, int P = 1>
+__global__ void __launch_bounds__(64 * W / P, K <= 4 ? 4 : 1)
     fluidx_x_kernel(T *__restrict__ u, const T *__restrict__ b, SoaBox bx, const T *__restrict__ dtp, int ntj,
                     T *__restrict__ cfl_part, double *__restrict__ mon_part)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    fluidx_x_tile<T, K, W, VEC, CFLM>(u, b, bx, dtp, xcd_tile(blockIdx.x, gridDim.x), ntj,
-                                      reinterpret_cast<T *>(smem_raw), cfl_part, nullptr, mon_part);
+    fluidx_x_tile<T, K, W, VEC, CFLM, P>(u, b, bx, dtp, xcd_tile(blockIdx.x, gridDim.x), ntj,
+                                         reinterpret_cast<T *>(smem_raw), cfl_part, nullptr, mon_part);
 }
 
 template <typename T, int K, int W, bool VEC>
@@ -247,6 +248,33 @@ template <typename T, int K, int FR> static hipError_t soa_attr()
     return e;
 }
 
+// several short pencils per wave (P > 1): the pencil length is (64/P)*K
+template <typename T, int K, int FR, int P> static hipError_t fluidx_p(T *u, const T *b, const SoaBox &bx, const T *dt,
+                                                                       hipStream_t st, T *cfl_part, int cflm,
+                                                                       double *mon_part)
+{
+    constexpr int W = soa_w<T>();
+    if constexpr (FR == 0) {
+        const int ntj = (bx.f.nj + W - 1) / W;
+        const size_t xl = sizeof(T) * fluidx_x_words<W>(bx.f.ni, cflm);
+        const dim3 grid(ntj * bx.f.nk), block(64 * W / P);
+#define MHD_FXP(C)                                                                                                   \
+    hipLaunchKernelGGL((fluidx_x_kernel<T, K, W, false, C, P>), grid, block, xl, st, u, b, bx, dt, ntj, cfl_part,    \
+                       mon_part)
+        if (cflm == 1) MHD_FXP(1);
+        else if (cflm == 2) MHD_FXP(2);
+        else MHD_FXP(0);
+#undef MHD_FXP
+    } else {
+        constexpr int W2 = 2 * W;
+        const int ntj2 = (bx.f.nj + W2 - 1) / W2;
+        const size_t lds2 = sizeof(T) * fluidx_soa_words<W2>(bx.f.ni);
+        hipLaunchKernelGGL((fluidx_soa_kernel<T, K, W2, FR, 0, P>), dim3(ntj2 * bx.f.nk), dim3(64 * W2 / P), lds2,
+                           st, u, b, bx, dt, ntj2);
+    }
+    return hipGetLastError();
+}
+
 template <typename T, int K, int FR>
 static hipError_t fluidx_soa_k(T *u, const T *b, const SoaBox &bx, const T *dt, hipStream_t st, T *cfl_part,
                                int cflm, double *mon_part)
@@ -332,6 +360,22 @@ hipError_t launch_fluidx_soa(T *u, const T *b, int fr, const SoaGrid &g, const T
 {
     if (cflm && fr != 0) return hipErrorInvalidValue;
     const SoaBox bx = make_soa_box<T>(fr, g);
+    if constexpr (sizeof(T) == 4) {
+        // short pencils that fill a wave P at a time (n = (64/P)*K), where
+        // that keeps K or the launch still has a wave per SIMD slot
+        const int P = p_pays(bx.f.ni, (long)bx.f.nj * bx.f.nk), KP = P > 1 ? bx.f.ni * P / 64 : 0;
+#define SFP(FR, KK, PP) return fluidx_p<T, KK, FR, PP>(u, b, bx, dt, st, cfl_part, cflm, mon_part)
+#define SFPF(FR)                                                                                                     \
+    if (P == 2 && KP == 1) SFP(FR, 1, 2);                                                                            \
+    if (P == 2 && KP == 3) SFP(FR, 3, 2);                                                                            \
+    if (P == 4 && KP == 1) SFP(FR, 1, 4);                                                                            \
+    if (P == 4 && KP == 3) SFP(FR, 3, 4);
+        if (fr == 0) { SFPF(0) }
+        else if (fr == 1) { SFPF(1) }
+        else { SFPF(2) }
+#undef SFPF
+#undef SFP
+    }
     const int K = pick_k(bx.f.ni);
     // any other pencil length: the generic kernels (no fused CFL)
     if (!K) return cflm ? hipErrorInvalidValue : launch_fluidx_gen<T>(u, b, bx.f, bx.N, 1, fr, dt, st);

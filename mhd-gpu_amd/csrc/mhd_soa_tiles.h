@@ -242,18 +242,21 @@ __device__ __forceinline__ void yz_row_scatter(const YzRow<T> &r, T *su, T *sb, 
     if (last) sb2n[i] = r.b2n;
 }
 
-// The tile's pencils from the LDS slots, one pencil per wave (fluidx.c:
-// 61-95): face B to cell centre, tvd1, the new u back into su.
-template <typename T, int K>
+// The tile's pencils from the LDS slots, one pencil per wave, or P per wave
+// on lane groups (fluidx.c:61-95): face B to cell centre, tvd1, the new u
+// back into su.
+template <typename T, int K, int P = 1>
 __device__ __forceinline__ void fluidx_lds_pencils(T *su, const T *sb, const T *sb2n, const T *sbk, int n, int ns,
                                                    const T *__restrict__ dtp)
 {
-    const int L = n / K;
-    const int w = threadIdx.x >> 6;
+    constexpr int G = 64 / P;
+    const int L = P == 1 ? n / K : G;
+    const int l = lane_id();
+    const int w = (threadIdx.x >> 6) * P + (P == 1 ? 0 : l / G);  // the lane's slot
     if (w < ns) {
-        const int l = lane_id();
-        const int lc = l % L;  // lanes >= L shadow lane l % L and store nothing
-        const int src_next = lc + 1 == L ? 0 : lc + 1, src_prev = lc == 0 ? L - 1 : lc - 1;
+        const int lc = P == 1 ? l % L : l & (G - 1);  // with P == 1, lanes >= L shadow lane l % L and store nothing
+        const int gb = P == 1 ? 0 : l & ~(G - 1);
+        const int src_next = gb + (lc + 1 == L ? 0 : lc + 1), src_prev = gb + (lc == 0 ? L - 1 : lc - 1);
         const T dt = *dtp;
         // the lane's K cells as 16-byte LDS reads (lane stride K values:
         // scalar reads of that stride collide in the banks)
@@ -281,8 +284,11 @@ __device__ __forceinline__ void fluidx_lds_pencils(T *su, const T *sb, const T *
                 bxc[q][2] = (T)0.5 * b3[q] + b3p[q] / (T)2;
             }
         }
-        tvd1_pencil<T, K>(uu, bxc, dt, src_next, src_prev);
-        if (l < L) {
+        if constexpr (P == 1)
+            tvd1_pencil<T, K>(uu, bxc, dt, LanesFull{src_next, src_prev});
+        else
+            tvd1_pencil<T, K>(uu, bxc, dt, LanesGroup<G>{src_next, src_prev});
+        if (l < L || P > 1) {
 #pragma unroll
             for (int m = 0; m < 5; m++) {
                 T x[K];
@@ -327,13 +333,14 @@ __device__ __forceinline__ void fluidx_yz_store(T *__restrict__ u, const T *su, 
 // step's last, so that is the next step's CFL
 // NF: the pencil length when the caller fixes it at compile time (n = 64K,
 // a full wave: the LDS offsets of the slots become immediates), else 0
-template <typename T, int K, int W, int FR, int CFLM = 0, int NF = 0>
+template <typename T, int K, int W, int FR, int CFLM = 0, int NF = 0, int P = 1>
 __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__restrict__ b, const SoaBox &bx,
                                                 const T *__restrict__ dtp, int tile, int ntj, T *smem,
                                                 T *__restrict__ cfl_part = nullptr)
 {
     static_assert(CFLM == 0 || FR == 0, "the CFL is fused into the x sweep only");
     static_assert(NF == 0 || NF == 64 * K, "a fixed pencil length fills the wave");
+    static_assert(P == 1 || (CFLM == 0 && NF == 0 && W % P == 0), "P pencils per wave: W/P waves, no fused CFL");
     constexpr int VE = Vec16<T>::n;
     static_assert(W == VE || (FR != 0 && W == 2 * VE), "tile width is one 16-byte vector (two in the y/z frames)");
     const Frame &f = bx.f;
@@ -426,7 +433,7 @@ __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__re
         }
     }
     __syncthreads();
-    fluidx_lds_pencils<T, K>(su, sb, sb2n, sbk, n, ns, dtp);
+    fluidx_lds_pencils<T, K, P>(su, sb, sb2n, sbk, n, ns, dtp);
     __syncthreads();
     if constexpr (CFLM != 0) {
         // CFL speed of the new state: after the barrier (a scheduling
@@ -887,27 +894,33 @@ __device__ __forceinline__ void advect_x_tile(const T *__restrict__ u, const T *
 // (b2 of pencil j+1, b3 at k+1) straight from global memory, b1 at i+1 by a
 // lane shuffle, and stores u back; no LDS and no barrier.  Same arithmetic
 // as fluidx_soa_tile.  CFLM as there: the u of the tile's pencils is kept
-// in LDS (5 n words per wave) for the CFL speed after a barrier.
+// in LDS (5 n words per pencil) for the CFL speed after a barrier.
+// P: pencils per wave (n = 64K/P, LanesGroup); the tile's W pencils then
+// take W/P waves, wave w holding pencils w*P .. w*P+P-1.
 // ---------------------------------------------------------------------------
 template <int W> __host__ __device__ constexpr int fluidx_x_words(int n, int cflm) { return cflm ? 5 * W * n : 0; }
 
-template <typename T, int K, int W, bool VEC, int CFLM>
+template <typename T, int K, int W, bool VEC, int CFLM, int P = 1>
 __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__restrict__ b, const SoaBox &bx,
                                               const T *__restrict__ dtp, int tile, int ntj, T *su,
                                               T *__restrict__ cfl_part, unsigned long long *cmax = nullptr,
                                               double *__restrict__ mon_part = nullptr)
 {
+    static_assert(P == 1 || (P == 2 || P == 4) && W % P == 0, "pencils per wave: 1, 2 or 4");
+    constexpr int G = 64 / P;
     const Frame &f = bx.f;
     const TileCtx t = tile_ctx<W>(bx, tile, ntj);
-    const int n = t.n, ns = t.ns, L = n / K;
+    const int n = t.n, ns = t.ns, L = P == 1 ? n / K : G;
     const long N = bx.N;
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int l = lane_id(), lc = l % L;
-    const int src_next = lc + 1 == L ? 0 : lc + 1, src_prev = lc == 0 ? L - 1 : lc - 1;
+    const int l = lane_id(), lc = P == 1 ? l % L : l & (G - 1);
+    const int ws = w * P + (P == 1 ? 0 : l / G);  // the lane's pencil in the tile
+    const int gb = P == 1 ? 0 : l & ~(G - 1);    // first lane of its group
+    const int src_next = gb + (lc + 1 == L ? 0 : lc + 1), src_prev = gb + (lc == 0 ? L - 1 : lc - 1);
     const long cell0 = (long)lc * K;
-    const long p = (long)(t.j0 + w) * f.sj + t.kb + cell0;                  // this pencil
-    const long pj = (long)wrapn(t.j0 + w + 1, f.nj) * f.sj + t.kb + cell0;  // pencil j+1
-    const long pk = (long)(t.j0 + w) * f.sj + t.kpb + cell0;                 // plane k+1
+    const long p = (long)(t.j0 + ws) * f.sj + t.kb + cell0;                  // this pencil
+    const long pj = (long)wrapn(t.j0 + ws + 1, f.nj) * f.sj + t.kb + cell0;  // pencil j+1
+    const long pk = (long)(t.j0 + ws) * f.sj + t.kpb + cell0;                 // plane k+1
     auto load = [&](const T *src, T (&o)[K]) {
         if constexpr (VEC) {
             load_cells<T, K>(src, o);
@@ -916,7 +929,7 @@ __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__rest
             for (int q = 0; q < K; q++) o[q] = src[q];
         }
     };
-    if (w < ns) {
+    if (ws < ns) {
         const T dt = *dtp;
         T uu[K][5], bxc[K][3];
         {
@@ -942,8 +955,11 @@ __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__rest
                 bxc[q][2] = (T)0.5 * b3[q] + b3p[q] / (T)2;
             }
         }
-        tvd1_pencil<T, K>(uu, bxc, dt, src_next, src_prev);
-        if (l < L) {
+        if constexpr (P == 1)
+            tvd1_pencil<T, K>(uu, bxc, dt, LanesFull{src_next, src_prev});
+        else
+            tvd1_pencil<T, K>(uu, bxc, dt, LanesGroup<G>{src_next, src_prev});
+        if (l < L || P > 1) {
 #pragma unroll
             for (int m = 0; m < 5; m++) {
                 T o[K];
@@ -955,7 +971,7 @@ __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__rest
 #pragma unroll
                     for (int q = 0; q < K; q++) u[m * N + p + q] = o[q];
                 }
-                if constexpr (CFLM != 0) store_cells<T, K>(su + (w * 5 + m) * n + cell0, o);
+                if constexpr (CFLM != 0) store_cells<T, K>(su + (ws * 5 + m) * n + cell0, o);
             }
         }
     }
@@ -969,7 +985,7 @@ __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__rest
         // momentum x, y, z, max |div B| on the left faces) of the tile's cells
         const bool mon = mon_part != nullptr;
         double ms[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        if (w < ns) {
+        if (ws < ns) {
             T b1[K], b2[K], b3[K];
             load(b + p, b1);
             load(b + N + p, b2);
@@ -982,7 +998,7 @@ __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__rest
             }
             T um[5][K];
 #pragma unroll
-            for (int m = 0; m < 5; m++) load_cells<T, K>(su + (w * 5 + m) * n + cell0, um[m]);
+            for (int m = 0; m < 5; m++) load_cells<T, K>(su + (ws * 5 + m) * n + cell0, um[m]);
 #pragma unroll
             for (int q = 0; q < K; q++) {
                 T B;
@@ -997,7 +1013,7 @@ __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__rest
                 const T sp = cfl_speed<T>(um[0][q], um[1][q], um[2][q], um[3][q], um[4][q], B);
                 c = cfl_max(c, sp);  // max_2num (calcfl.c:55)
             }
-            if (mon && l < L) {
+            if (mon && (l < L || P > 1)) {
 #pragma unroll
                 for (int q = 0; q < K; q++) {
                     ms[0] += (double)um[0][q];
