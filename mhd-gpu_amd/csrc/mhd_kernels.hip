@@ -10,7 +10,12 @@
 // padded to an odd word count so that lane-strided reads are conflict free.
 // Neighbours along the pencil come from lane shuffles; the pencil-global max
 // of the freezing speed is a wave reduction.
+#include <stdint.h>
+
+#include <initializer_list>
+
 #include "mhd_kernels.h"
+#include "mhd_aos_x.h"
 #include "mhd_core.h"
 #include "mhd_generic.h"
 
@@ -691,32 +696,84 @@ template <typename T> hipError_t prepare_sweeps(int nx, int ny, int nz)
     return e;
 }
 
+// the widest record loads the x-frame AoS kernels may use: 16-byte loads
+// when every lane's first record (cell nx*(j + ny*k) + l*K) and the arrays
+// are 16-byte aligned (nx and K multiples of 4), 8-byte loads when they are
+// 8-byte aligned, else one value at a time
+static int aos_vec(int nx, int K, std::initializer_list<const void *> ptrs)
+{
+    bool a16 = true, a8 = true;
+    for (const void *p : ptrs) {
+        a16 = a16 && ((uintptr_t)p % 16 == 0);
+        a8 = a8 && ((uintptr_t)p % 8 == 0);
+    }
+    if (sizeof(float) == 4 && a16 && nx % 4 == 0 && K % 4 == 0) return 4;
+    if (a8 && nx % 2 == 0 && K % 2 == 0) return 2;
+    return 1;
+}
+
 template <typename T, int K, int FR>
 static hipError_t fluidx_launch_k(T *u, const T *b, const Frame &f, const T *dt, hipStream_t st)
 {
     constexpr int W = tile_w<T>();
-    const size_t lds = sizeof(T) * fluidx_lds_words<T, K, W>(f.ni);
-    auto kern = fluidx_kernel<T, K, W, FR>;
-    const hipError_t ea = sweep_attr<T, K, FR, 0>();
-    if (ea != hipSuccess) return ea;
-    const int ntj = (f.nj + W - 1) / W;
-    const int nblk = ntj * f.nk;
-    hipLaunchKernelGGL(kern, dim3(nblk), dim3(64 * W), lds, st, u, b, f, dt, ntj);
-    return hipGetLastError();
+    if constexpr (FR == 0) {
+        // along x: whole records straight from global memory (mhd_aos_x.h)
+        const int ntj = (f.nj + W - 1) / W;
+        const dim3 grid(ntj * f.nk), block(64 * W);
+        const int v = aos_vec(f.ni, K, {u, b});
+        if (v == 4 && sizeof(T) == 4)
+            hipLaunchKernelGGL((fluidx_aos_x_kernel<T, K, W, 4>), grid, block, 0, st, u, b, f, dt, ntj);
+        else if (v >= 2)
+            hipLaunchKernelGGL((fluidx_aos_x_kernel<T, K, W, 2>), grid, block, 0, st, u, b, f, dt, ntj);
+        else
+            hipLaunchKernelGGL((fluidx_aos_x_kernel<T, K, W, 1>), grid, block, 0, st, u, b, f, dt, ntj);
+        return hipGetLastError();
+    } else {
+        const size_t lds = sizeof(T) * fluidx_lds_words<T, K, W>(f.ni);
+        auto kern = fluidx_kernel<T, K, W, FR>;
+        const hipError_t ea = sweep_attr<T, K, FR, 0>();
+        if (ea != hipSuccess) return ea;
+        const int ntj = (f.nj + W - 1) / W;
+        const int nblk = ntj * f.nk;
+        hipLaunchKernelGGL(kern, dim3(nblk), dim3(64 * W), lds, st, u, b, f, dt, ntj);
+        return hipGetLastError();
+    }
 }
 
 template <typename T, int K, int FR>
 static hipError_t advect_launch_k(const T *u, const T *bin, T *bout, const Frame &f, const T *dt, hipStream_t st)
 {
     constexpr int W = tile_w<T>();
-    const size_t lds = sizeof(T) * advect_lds_words<T, K, W>(f.ni);
-    auto kern = advect_kernel<T, K, W, FR>;
-    const hipError_t ea = sweep_attr<T, K, FR, 1>();
-    if (ea != hipSuccess) return ea;
-    const int ntj = (f.nj + W - 1) / W;
-    const int nblk = ntj * f.nk;
-    hipLaunchKernelGGL(kern, dim3(nblk), dim3(64 * W), lds, st, u, bin, bout, f, dt, ntj);
-    return hipGetLastError();
+    if constexpr (FR == 0) {
+        // along x: tiles of two planes with direct record loads (mhd_aos_x.h)
+        const int ntj = (f.nj + W - 1) / W;
+        const dim3 grid(ntj * ((f.nk + 1) / 2)), block(64 * W);
+        const size_t lds = sizeof(T) * advect_aos_x2_words<W>(f.ni);
+        const int v = aos_vec(f.ni, K, {u, bin, bout});
+        hipError_t e = hipSuccess;
+#define MHD_AX(V)                                                                                                    \
+    do {                                                                                                             \
+        static const hipError_t a = hipFuncSetAttribute((const void *)advect_aos_x2_kernel<T, K, W, V>,              \
+                                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);    \
+        e = a;                                                                                                       \
+        if (e == hipSuccess)                                                                                         \
+            hipLaunchKernelGGL((advect_aos_x2_kernel<T, K, W, V>), grid, block, lds, st, u, bin, bout, f, dt, ntj); \
+    } while (0)
+        if (v == 4 && sizeof(T) == 4) MHD_AX(4);
+        else if (v >= 2) MHD_AX(2);
+        else MHD_AX(1);
+#undef MHD_AX
+        return e != hipSuccess ? e : hipGetLastError();
+    } else {
+        const size_t lds = sizeof(T) * advect_lds_words<T, K, W>(f.ni);
+        auto kern = advect_kernel<T, K, W, FR>;
+        const hipError_t ea = sweep_attr<T, K, FR, 1>();
+        if (ea != hipSuccess) return ea;
+        const int ntj = (f.nj + W - 1) / W;
+        const int nblk = ntj * f.nk;
+        hipLaunchKernelGGL(kern, dim3(nblk), dim3(64 * W), lds, st, u, bin, bout, f, dt, ntj);
+        return hipGetLastError();
+    }
 }
 
 
