@@ -282,12 +282,10 @@ static int transpose_common(int which, float *ut, float *bt, float *u, float *b,
 {
     if (!ut || !bt || !u || !b || !nx || !ny || !nz) return fail(MHD_ERR_ARG, "mhd_transpose: null argument");
     if (*nx < 1 || *ny < 1 || *nz < 1) return fail(MHD_ERR_SHAPE, "mhd_transpose: bad grid");
-    const size_t nc = (size_t)*nx * *ny * *nz;
-    hipError_t e = launch_transpose<float>(ut, bt, u, b, which, *nx, *ny, *nz, 0);
-    if (e != hipSuccess) return hip_fail(e, "mhd_transpose launch");
-    // copy back so that u,b hold the new frame (HIP/cu/cuda_transpose12.cu:33-34)
-    e = hipMemcpyAsync(u, ut, nc * 5 * sizeof(float), hipMemcpyDeviceToDevice, 0);
-    if (e == hipSuccess) e = hipMemcpyAsync(b, bt, nc * 3 * sizeof(float), hipMemcpyDeviceToDevice, 0);
+    // the new frame into ut,bt and into u,b, as the reference's transpose
+    // and copy back leave them (HIP/cu/cuda_transpose12.cu:33-34): in place
+    // for square planes, else out of place and copied
+    const hipError_t e = launch_transpose<float>(ut, bt, u, b, which, *nx, *ny, *nz, 0, true);
     return sync_check(e, "mhd_transpose");
 }
 

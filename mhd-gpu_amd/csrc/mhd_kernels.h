@@ -54,7 +54,8 @@ hipError_t launch_rcp_scan(unsigned long long *maxbits, hipStream_t st);
 
 // which = 12 or 13: out-of-place transposes with the reference's component maps
 template <typename T>
-hipError_t launch_transpose(T *uo, T *bo, const T *u, const T *b, int which, int nx, int ny, int nz, hipStream_t st);
+hipError_t launch_transpose(T *uo, T *bo, const T *u, const T *b, int which, int nx, int ny, int nz, hipStream_t st,
+                            bool both = false);
 
 // deterministic reduction of np monitor partial records (6 doubles each:
 // mass, energy, momentum x, y, z, max |div B|) into out[6] (when not null)

@@ -546,6 +546,86 @@ __global__ void __launch_bounds__(256) transpose_kernel(T *__restrict__ out, con
 }
 
 // ---------------------------------------------------------------------------
+// The same permutation into two buffers at once, for square planes (na == nb,
+// so sao == sbi and sco == sci): the transposed tile (A,B) lands exactly on
+// the input positions of tile (B,A).  One block reads both tiles of a pair
+// (A <= B) of plane c3 into LDS, then writes each transposed onto the other's
+// positions, in io (the input, updated in place: the reference's copy back,
+// HIP/cu/cuda_transpose12.cu:33-34) and in out.  No other block touches those
+// positions.  VEC: whole 32-cell rows as 16-byte vectors (n % 32 == 0, every
+// row 16-byte aligned); otherwise one value at a time with bounds checks.
+// ---------------------------------------------------------------------------
+template <typename T, int C, bool VEC>
+__global__ void __launch_bounds__(256) transpose_pair_kernel(T *__restrict__ out, T *io, int n, int nt, long sb,
+                                                             long sc, int p0, int p1, int p2, int p3, int p4)
+{
+    constexpr int PITCH = 32 * C + 4;  // row pitch in values (16-byte rows for the vector path)
+    __shared__ __align__(16) T tile[2][32][PITCH];
+    const int perm[5] = {p0, p1, p2, p3, p4};
+    // pair index -> (A, B), A <= B, in the upper triangle of nt x nt tiles
+    const int pi = blockIdx.x, c3 = blockIdx.y;
+    int B = (int)((sqrtf(8.0f * pi + 1.0f) - 1.0f) * 0.5f);
+    while ((B + 1) * (B + 2) / 2 <= pi) B++;
+    while (B * (B + 1) / 2 > pi) B--;
+    const int A = pi - B * (B + 1) / 2;
+    const int ntile = A == B ? 1 : 2;
+    const long pl = (long)c3 * sc;
+    const int tid = threadIdx.x;
+    for (int tt = 0; tt < ntile; tt++) {
+        const int a0 = (tt == 0 ? A : B) * 32, b0 = (tt == 0 ? B : A) * 32;
+        if constexpr (VEC && sizeof(T) == 4) {
+            constexpr int RV = 8 * C;  // 16-byte vectors per 32-cell row
+            for (int e = tid; e < 32 * RV; e += 256) {
+                const int bb = e / RV, v = e - bb * RV;
+                const float4 x =
+                    reinterpret_cast<const float4 *>(io + ((long)a0 + (long)(b0 + bb) * sb + pl) * C)[v];
+                *reinterpret_cast<float4 *>(&tile[tt][bb][4 * v]) = x;
+            }
+        } else {
+            for (int e = tid; e < 32 * 32 * C; e += 256) {
+                const int bb = e / (32 * C), r = e - bb * (32 * C);
+                const int aa = r / C;
+                if (a0 + aa < n && b0 + bb < n)
+                    tile[tt][bb][r] = io[((long)(a0 + aa) + (long)(b0 + bb) * sb + pl) * C + (r - aa * C)];
+            }
+        }
+    }
+    __syncthreads();
+    for (int tt = 0; tt < ntile; tt++) {
+        // tile tt holds a in [a0, a0+32), b in [b0, b0+32); its transpose
+        // goes to rows a (output cell b + a*sb), b contiguous
+        const int a0 = (tt == 0 ? A : B) * 32, b0 = (tt == 0 ? B : A) * 32;
+        if constexpr (VEC && sizeof(T) == 4) {
+            constexpr int RV = 8 * C;
+            for (int e = tid; e < 32 * RV; e += 256) {
+                const int aa = e / RV, v = e - aa * RV;
+                float y[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int f = 4 * v + i, bb = f / C, q = f - bb * C;
+                    y[i] = tile[tt][bb][aa * C + perm[q]];
+                }
+                const long o = ((long)b0 + (long)(a0 + aa) * sb + pl) * C;
+                const float4 x = make_float4(y[0], y[1], y[2], y[3]);
+                reinterpret_cast<float4 *>(out + o)[v] = x;
+                reinterpret_cast<float4 *>(io + o)[v] = x;
+            }
+        } else {
+            for (int e = tid; e < 32 * 32 * C; e += 256) {
+                const int aa = e / (32 * C), r = e - aa * (32 * C);
+                const int bb = r / C, q = r - bb * C;
+                if (a0 + aa < n && b0 + bb < n) {
+                    const long o = ((long)(b0 + bb) + (long)(a0 + aa) * sb + pl) * C + q;
+                    const T x = tile[tt][bb][aa * C + perm[q]];
+                    out[o] = x;
+                    io[o] = x;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // FP64 monitors: the deterministic reduction of np partial records (mass,
 // energy, momentum x, y, z summed; max |div B| max-ed; 6 doubles each).
 // Thread t takes records t, t+nt, ... in order, then a fixed tree over the
@@ -861,7 +941,8 @@ hipError_t launch_zero_u64(unsigned long long *p, int n, hipStream_t st)
 }
 
 template <typename T>
-hipError_t launch_transpose(T *uo, T *bo, const T *u, const T *b, int which, int nx, int ny, int nz, hipStream_t st)
+hipError_t launch_transpose(T *uo, T *bo, const T *u, const T *b, int which, int nx, int ny, int nz, hipStream_t st,
+                            bool both)
 {
     // which 12: (i,j,k)->(j,i,k), u (1,3,2,4,5), b (2,1,3)
     // which 13: (i,j,k)->(k,j,i), u (1,4,3,2,5), b (3,2,1)
@@ -879,12 +960,39 @@ hipError_t launch_transpose(T *uo, T *bo, const T *u, const T *b, int which, int
         for (int i = 0; i < 5; i++) pu[i] = a[i];
         for (int i = 0; i < 3; i++) pb[i] = c[i];
     }
+    if (both && na == nb) {
+        // square planes: transpose in place into u, b and into uo, bo at once
+        const int nt = (na + 31) / 32;
+        const dim3 grid(nt * (nt + 1) / 2, nc);
+        auto aligned = [](const void *p) { return (uintptr_t)p % 16 == 0; };
+        const bool vec = sizeof(T) == 4 && na % 32 == 0 && aligned(u) && aligned(b) && aligned(uo) && aligned(bo);
+        T *iu = const_cast<T *>(u), *ib = const_cast<T *>(b);
+        if (vec) {
+            hipLaunchKernelGGL((transpose_pair_kernel<T, 5, true>), grid, dim3(256), 0, st, uo, iu, na, nt, sbi, sci,
+                               pu[0], pu[1], pu[2], pu[3], pu[4]);
+            hipLaunchKernelGGL((transpose_pair_kernel<T, 3, true>), grid, dim3(256), 0, st, bo, ib, na, nt, sbi, sci,
+                               pb[0], pb[1], pb[2], 0, 0);
+        } else {
+            hipLaunchKernelGGL((transpose_pair_kernel<T, 5, false>), grid, dim3(256), 0, st, uo, iu, na, nt, sbi, sci,
+                               pu[0], pu[1], pu[2], pu[3], pu[4]);
+            hipLaunchKernelGGL((transpose_pair_kernel<T, 3, false>), grid, dim3(256), 0, st, bo, ib, na, nt, sbi, sci,
+                               pb[0], pb[1], pb[2], 0, 0);
+        }
+        return hipGetLastError();
+    }
     dim3 grid((na + 31) / 32, (nb + 31) / 32, nc);
     hipLaunchKernelGGL((transpose_kernel<T, 5>), grid, dim3(256), 0, st, uo, u, na, nb, sbi, sci, sao, sco, pu[0],
                        pu[1], pu[2], pu[3], pu[4]);
     hipLaunchKernelGGL((transpose_kernel<T, 3>), grid, dim3(256), 0, st, bo, b, na, nb, sbi, sci, sao, sco, pb[0],
                        pb[1], pb[2], 0, 0);
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (both && e == hipSuccess) {
+        // the copy back of the reference (HIP/cu/cuda_transpose12.cu:33-34)
+        const size_t ncell = (size_t)nx * ny * nz;
+        e = hipMemcpyAsync(const_cast<T *>(u), uo, ncell * 5 * sizeof(T), hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(const_cast<T *>(b), bo, ncell * 3 * sizeof(T), hipMemcpyDeviceToDevice, st);
+    }
+    return e;
 }
 
 // first stage of a large reduction: block g reduces records [256g, 256g+256)
@@ -933,7 +1041,7 @@ hipError_t launch_monitor_reduce(const double *part, int np, double *out, double
     template hipError_t launch_fill_zero<T>(T *, int, hipStream_t);                                                 \
     template hipError_t launch_cfl_value<T>(unsigned long long *, T *, const T *, const T *, int, int, int, int,     \
                                             hipStream_t);                                                           \
-    template hipError_t launch_transpose<T>(T *, T *, const T *, const T *, int, int, int, int, hipStream_t);       \
+    template hipError_t launch_transpose<T>(T *, T *, const T *, const T *, int, int, int, int, hipStream_t, bool); \
     template hipError_t prepare_sweeps<T>(int, int, int);
 MHD_INST(float)
 MHD_INST(double)

@@ -101,17 +101,23 @@ def test_single_ops_on_reference_fixtures(M, oracle, golden, fixture16, fixture3
     assert same(ug, uo)
 
 
-@pytest.mark.parametrize("shape", [(6, 5, 4), (32, 32, 32), (33, 17, 9), (40, 3, 70)])
+# square planes (nx == ny for transpose12, nx == nz for transpose13) take the
+# in-place pair swap, with 16-byte rows where nx is a multiple of 32 (32^3,
+# 96^3, 64x32x64) and value by value elsewhere (40x40x7, 20x9x20); the other
+# shapes transpose out of place and copy back
+@pytest.mark.parametrize("shape", [(6, 5, 4), (32, 32, 32), (33, 17, 9), (40, 3, 70), (96, 96, 96), (64, 32, 64),
+                                   (40, 40, 7), (20, 9, 20)])
 def test_transposes_match_oracle(M, oracle, shape):
     nx, ny, nz = shape
     u0, b0 = random_state(nx, ny, nz, seed=300)
     for which in (12, 13):
+        print("transpose", which, shape, flush=True)
         du, db = M.DeviceArray.from_host(u0), M.DeviceArray.from_host(b0)
         ut, bt = M.DeviceArray(u0.nbytes), M.DeviceArray(b0.nbytes)
         getattr(M, f"transpose{which}")(ut, bt, du, db, nx, ny, nz)
         uo, bo = getattr(oracle, f"transpose{which}")(u0, b0, nx, ny, nz)
         assert bitwise(du.to_host(), uo) and bitwise(db.to_host(), bo)
-        assert bitwise(ut.to_host(), uo)
+        assert bitwise(ut.to_host(), uo) and bitwise(bt.to_host(), bo)
 
 
 @pytest.mark.parametrize("shape,avg", [((16, 16, 16), False), ((33, 8, 5), False), ((24, 12, 6), True),
