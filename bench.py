@@ -292,7 +292,7 @@ def precision_report(mhdgpu, n, steps, ic, seed, state=None):
     return out
 
 
-def steplevel_report(mhdgpu, n, steps, warmup):
+def steplevel_report(mhdgpu, n, steps, warmup, state=None):
     """The reference-shaped path: the loop of HIP/cu/cuda_main.cu:89-121 driven
     through the step-level entry points on AoS device arrays (mhd_cfl, then
     dt = 0.9*cfl clamped as Heterogeneous_MHD/C/mhd.c:63-65 on the host, then
@@ -300,7 +300,7 @@ def steplevel_report(mhdgpu, n, steps, warmup):
     write ut, bt and copy back).  Every entry point is synchronous, like the
     reference's wrappers; per-op times are host wall times around each call."""
     from mhdgpu.ics import random_phase
-    u0, b0 = random_phase(n, seed=2)
+    u0, b0 = state if state is not None else random_phase(n, seed=2)
     du, db = mhdgpu.DeviceArray.from_host(u0), mhdgpu.DeviceArray.from_host(b0)
     dut, dbt = mhdgpu.DeviceArray(u0.nbytes), mhdgpu.DeviceArray(b0.nbytes)
     dcfl, ddt = mhdgpu.DeviceArray(4), mhdgpu.DeviceArray(4)
@@ -357,6 +357,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-steplevel", action="store_true")
     ap.add_argument("--requests", type=int, default=64)
     ap.add_argument("--mode", choices=["plain", "copy", "precision", "steplevel"], default="plain",
                     help="copy: time the plain copy kernel on 1 GiB (counter calibration); "
@@ -481,6 +482,10 @@ def main():
                            "zone_updates_per_s": n ** 3 * args.steps / wall_main,
                            "includes": "device allocation, H2D of u and b, the steps, D2H (PCIe-inclusive)"}
     del uh, bh
+    if not args.no_steplevel:
+        # the reference's own call sequence through the step-level entry points
+        log("bench: step-level loop, 5 steps")
+        out["steplevel"] = steplevel_report(mhdgpu, n, 5, 1, state=(u0, b0))
     if not args.no_batched:
         out["batched"] = bench_batched(mhdgpu)
     if not args.no_served:

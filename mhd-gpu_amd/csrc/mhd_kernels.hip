@@ -825,7 +825,10 @@ static hipError_t advect_launch_k(const T *u, const T *bin, T *bout, const Frame
 {
     constexpr int W = tile_w<T>();
     if constexpr (FR == 0) {
-        // along x: tiles of two planes with direct record loads (mhd_aos_x.h)
+        // along x: tiles of two planes with direct record loads (mhd_aos_x.h),
+        // eight pencils wide (measured 4% faster than four at 256^3: fewer
+        // halo pencils per pencil)
+        constexpr int W = 2 * tile_w<T>();
         const int ntj = (f.nj + W - 1) / W;
         const dim3 grid(ntj * ((f.nk + 1) / 2)), block(64 * W);
         const size_t lds = sizeof(T) * advect_aos_x2_words<W>(f.ni);
