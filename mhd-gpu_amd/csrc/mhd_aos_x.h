@@ -200,7 +200,7 @@ __global__ void __launch_bounds__(64 * W)
         shift_prev<T, K>(vx, vxm, src_prev);
         shift_next<T, K>(vx, vxn, src_next);
 #pragma unroll
-        for (int q = 0; q < K; q++) vs[q] = (T)(((double)(vxm[q] + vxn[q]) + 2.0 * (double)vx[q]) / 4.0);
+        for (int q = 0; q < K; q++) vs[q] = avg121<T>(vxm[q] + vxn[q], vx[q]);
         tvdb_pencil<T, K>(bi, vs, dt, bnew, F, src_next, src_prev);
         if (l < L) {
             store_cells<T, K>(X + it * n + cell0, F);

@@ -156,11 +156,65 @@ template <typename T> __device__ __forceinline__ T vanleer(T p, T m)
     return d;
 }
 
-// ps of mhdflux.c:28 / calcfl.c:50, S = u2^2+u3^2+u4^2 and B = b1^2+b2^2+b3^2 in T
-template <typename T> __device__ __forceinline__ T pstar(T u1, T u5, T S, T B)
+// Exact shortcuts of three FP64 sub-expressions of the reference's FP32 code
+// (double literals promote them).  For FP32 operands:
+//  * (float)((double)x / 2.0): the halving is exact in double, so this is
+//    x/2 rounded once to float, which the FP32 product x*0.5f also is
+//    (FP32 denormals are kept): one operation instead of three;
+//  * (float)((double)a - (double)x / 2.0): x/2 is exact, so the difference
+//    is one fma;
+//  * (float)(((double)s + 2.0*(double)v) / 4.0): 2v is exact, so the sum
+//    is one fma, and /4 is exact.
+// The FP64 instantiation keeps the literal expressions.
+template <typename T> __device__ __forceinline__ T half_of(T x)
+{
+    if constexpr (sizeof(T) == 4)
+        return x * 0.5f;
+    else
+        return (T)((double)x / 2.0);
+}
+template <typename T> __device__ __forceinline__ T sub_half(T a, T x)
+{
+    if constexpr (sizeof(T) == 4)
+        return (T)fma((double)x, -0.5, (double)a);
+    else
+        return (T)((double)a - (double)x / 2.0);
+}
+template <typename T> __device__ __forceinline__ T avg121(T s, T v)
+{
+    if constexpr (sizeof(T) == 4)
+        return (T)(fma((double)v, 2.0, (double)s) * 0.25);
+    else
+        return (T)(((double)s + 2.0 * (double)v) / 4.0);
+}
+
+// ps of mhdflux.c:28 / calcfl.c:50 from q = S/u1 (S = u2^2+u3^2+u4^2) and
+// B = b1^2+b2^2+b3^2 in T:
+//   ((double)u5 - (double)q/2.0)*((double)g - 1.0) + (2.0 - (double)g)*(double)B/2.0
+// For FP32 two steps take one operation each instead of two, with the same
+// result: (double)q/2.0 is exact (a float value halves exactly in double),
+// so u5 - q/2 is one fma; and (2 - g) has 24 significant bits, so
+// (2 - g)*B is exact in double and halving it is exact, which makes it
+// B*((2 - g)/2), one product.
+template <typename T> __device__ __forceinline__ T pstar_q(T q, T u5, T B)
 {
     const T g = gamma_c<T>();
-    return (T)(((double)u5 - (double)(S / u1) / 2.0) * ((double)g - 1.0) + (2.0 - (double)g) * (double)B / 2.0);
+    if constexpr (sizeof(T) == 4) {
+        const double t = fma((double)q, -0.5, (double)u5);
+        return (T)(t * ((double)g - 1.0) + (double)B * ((2.0 - (double)g) * 0.5));
+    } else {
+        return (T)(((double)u5 - (double)q / 2.0) * ((double)g - 1.0) + (2.0 - (double)g) * (double)B / 2.0);
+    }
+}
+template <typename T> __device__ __forceinline__ T pstar(T u1, T u5, T S, T B) { return pstar_q<T>(S / u1, u5, B); }
+// p = ps - B/2.0 in double (mhdflux.c:37, calcfl.c:52); for FP32 B/2.0 is
+// exact, so the difference is one fma
+template <typename T> __device__ __forceinline__ T pgas(T ps, T B)
+{
+    if constexpr (sizeof(T) == 4)
+        return (T)fma((double)B, -0.5, (double)ps);
+    else
+        return (T)((double)ps - (double)B / 2.0);
 }
 
 // One cell of mhdflux (mhdflux.c:21-46): unnormalised flux v[5] and the
@@ -179,7 +233,7 @@ __device__ __forceinline__ void mhdflux_cell(const T *u, const T *b, T *v, T &ci
     v[2] = u3 * vx - b2 * b1;
     v[3] = u4 * vx - b3 * b1;
     v[4] = (u5 + ps) * vx - b1 * (b1 * u2 + b2 * u3 + b3 * u4) / u1;
-    const T p = (T)((double)ps - (double)B / 2.0);
+    const T p = pgas<T>(ps, B);
     const T g = gamma_c<T>();
     ci = (T)(fabs((double)vx) + sqrt_of<T>(fabs((double)((B + g * p) / u1))));
 }
@@ -226,15 +280,14 @@ __device__ __forceinline__ void mhdflux_cell_bulk(const float *u, const float *b
     const float S = u2 * u2 + u3 * u3 + u4 * u4;
     const float B = b1 * b1 + b2 * b2 + b3 * b3;
     const float g = gamma_c<float>();
-    const float ps = (float)(((double)u5 - (double)quot_by(S, r1) / 2.0) * ((double)g - 1.0) +
-                             (2.0 - (double)g) * (double)B / 2.0);
+    const float ps = pstar_q<float>(quot_by(S, r1), u5, B);
     v[0] = u2;
     v[1] = u2 * vx + ps - b1 * b1;
     v[2] = u3 * vx - b2 * b1;
     v[3] = u4 * vx - b3 * b1;
     const float bm = b1 * (b1 * u2 + b2 * u3 + b3 * u4);
     v[4] = (u5 + ps) * vx - quot_by(bm, r1);
-    const float p = (float)((double)ps - (double)B / 2.0);
+    const float p = pgas<float>(ps, B);
     const float w = B + g * p;
     ci = (float)(fabs((double)vx) + sqrt_ranged(fabs((double)quot_by(w, r1))));
     vmax = fmaxf(fmaxf(vmax, fabsf(u2)), fmaxf(fmaxf(fabsf(S), fabsf(bm)), fabsf(w)));
@@ -267,8 +320,10 @@ __device__ __forceinline__ T cfl_speed(T u1, T u2, T u3, T u4, T u5, T B)
     }
     const T S = u2 * u2 + u3 * u3 + u4 * u4;
     const T ps = pstar<T>(u1, u5, S, B);
-    const T p = (T)((double)ps - (double)B / 2.0);
-    return (T)((double)v + sqrt_of<T>(fabs((double)B * 2.0 + (double)(g * p)) / (double)u1));
+    const T p = pgas<T>(ps, B);
+    // B*2.0 + gamma*p in double; for FP32 2B is exact, so the sum is one fma
+    const double num = sizeof(T) == 4 ? fma((double)B, 2.0, (double)(g * p)) : (double)B * 2.0 + (double)(g * p);
+    return (T)((double)v + sqrt_of<T>(fabs(num) / (double)u1));
 }
 
 // ---- wave-level helpers: a pencil of n = L*K cells lives in one wave, lane
@@ -530,9 +585,9 @@ template <typename T, bool AVG> struct AosSpeed {
             const long sip = s - i + (i + 1 == nx ? 0 : i + 1);
             const long sjp = ((long)k * ny + (j + 1 == ny ? 0 : j + 1)) * nx + i;
             const long skp = ((long)(k + 1 == nz ? 0 : k + 1) * ny + j) * nx + i;
-            const T bx = (T)((double)(pb[0] + b[3 * sip + 0]) / 2.0);
-            const T by = (T)((double)(pb[1] + b[3 * sjp + 1]) / 2.0);
-            const T bz = (T)((double)(pb[2] + b[3 * skp + 2]) / 2.0);
+            const T bx = half_of<T>(pb[0] + b[3 * sip + 0]);
+            const T by = half_of<T>(pb[1] + b[3 * sjp + 1]);
+            const T bz = half_of<T>(pb[2] + b[3 * skp + 2]);
             B = bx * bx + by * by + bz * bz;
         } else {
             B = pb[0] * pb[0] + pb[1] * pb[1] + pb[2] * pb[2];
@@ -554,9 +609,9 @@ template <typename T, bool AVG> struct SoaSpeed {
             const long sip = row + (i + 1 == g.nx ? 0 : i + 1);
             const long sjp = (long)(j + 1 == g.ny ? 0 : j + 1) * g.sy + (long)k * g.sz + i;
             const long skp = (long)j * g.sy + (long)(k + 1 == g.nz ? 0 : k + 1) * g.sz + i;
-            const T bx = (T)((double)(b[c] + b[sip]) / 2.0);
-            const T by = (T)((double)(b[N + c] + b[N + sjp]) / 2.0);
-            const T bz = (T)((double)(b[2 * N + c] + b[2 * N + skp]) / 2.0);
+            const T bx = half_of<T>(b[c] + b[sip]);
+            const T by = half_of<T>(b[N + c] + b[N + sjp]);
+            const T bz = half_of<T>(b[2 * N + c] + b[2 * N + skp]);
             B = bx * bx + by * by + bz * bz;
         } else {
             B = b[c] * b[c] + b[N + c] * b[N + c] + b[2 * N + c] * b[2 * N + c];

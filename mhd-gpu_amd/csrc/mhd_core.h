@@ -216,7 +216,7 @@ __device__ __forceinline__ void tvdb_pencil(const T (&bin)[K], const T (&vg)[K],
     shift_prev<T, K>(f1, f1p, ln);
 #pragma unroll
     for (int q = 0; q < K; q++) {
-        b1[q] = (T)((double)bin[q] - (double)((f1[q] - f1p[q]) * dt) / 2.0);
+        b1[q] = sub_half<T>(bin[q], (f1[q] - f1p[q]) * dt);
         g[q] = vg[q] * b1[q];
     }
     shift_next<T, K>(g, gn, ln);
@@ -227,12 +227,12 @@ __device__ __forceinline__ void tvdb_pencil(const T (&bin)[K], const T (&vg)[K],
         T w, wp, wm;
         if (vh[q] > (T)0) {
             w = g[q];
-            wp = (T)((double)(gn[q] - w) / 2.0);
-            wm = (T)((double)(w - gp[q]) / 2.0);
+            wp = half_of<T>(gn[q] - w);
+            wm = half_of<T>(w - gp[q]);
         } else {
             w = gn[q];
-            wp = (T)((double)(w - gnn[q]) / 2.0);
-            wm = (T)((double)(g[q] - w) / 2.0);
+            wp = half_of<T>(w - gnn[q]);
+            wm = half_of<T>(g[q] - w);
         }
         const T dw = vanleer<T>(wm, wp);
         fl[q] = (w + dw) * dt;

@@ -292,7 +292,7 @@ __global__ void __launch_bounds__(64 * W) advect_kernel(const T *__restrict__ u,
 #pragma unroll
         for (int q = 0; q < K; q++) {
             // A1.c:41 (vx[i-1] + vx[i+1] + 2.0*vx)/4.0
-            vs[q] = (T)(((double)(vxm[q] + vxn[q]) + 2.0 * (double)vx[q]) / 4.0);
+            vs[q] = avg121<T>(vxm[q] + vxn[q], vx[q]);
             const int i = lc * K + q;
             if (fkind == 0)
                 bin[q] = sb[fslot * PB + pidx<K, 3>(i, 1)];

@@ -45,7 +45,7 @@ template <typename T> struct TvdbStream {
     {
         const T vr0 = face_vel(ma, mb, ra, rb);
         // A1.c:41 vx = (vx[i-1] + vx[i+1] + 2.0*vx)/4.0 at i = t-1
-        const T vg1 = (T)(((double)(vr2 + vr0) + 2.0 * (double)vr1) / 4.0);
+        const T vg1 = avg121<T>(vr2 + vr0, vr1);
         // tvdb.c:27 vh = 0.5*vg + 0.5*vg[i+1] at i = t-2; tvdb.c:33 b*vg at t-1
         const bool up2 = (T)0.5 * vg2 + (T)0.5 * vg1 > (T)0;
         const T bv1 = bw1 * vg1;
@@ -54,18 +54,18 @@ template <typename T> struct TvdbStream {
         const T bv2 = bw2 * vg2;
         const T f12 = up2 ? bv2 : bv1;
         // tvdb.c:51 b1 = b - (flux1 - flux1[i-1])*dt/2.0 at t-2
-        const T b12 = (T)((double)bw2 - (double)((f12 - f13) * dt) / 2.0);
+        const T b12 = sub_half<T>(bw2, (f12 - f13) * dt);
         const T g2 = vg2 * b12;
         // tvdb.c:54-78 limited flux at i = t-4 (g[i-1..i+2] = g5, g4, g3, g2)
         T w, wp, wm;
         if (up4) {
             w = g4;
-            wp = (T)((double)(g3 - w) / 2.0);
-            wm = (T)((double)(w - g5) / 2.0);
+            wp = half_of<T>(g3 - w);
+            wm = half_of<T>(w - g5);
         } else {
             w = g3;
-            wp = (T)((double)(w - g2) / 2.0);
-            wm = (T)((double)(g4 - w) / 2.0);
+            wp = half_of<T>(w - g2);
+            wm = half_of<T>(g4 - w);
         }
         const T dw = vanleer<T>(wm, wp);
         const T fl4 = (w + dw) * dt;
@@ -82,12 +82,12 @@ template <typename T> struct TvdbStream {
     __device__ __forceinline__ void push_fill(T ma, T mb, T ra, T rb, T bt, T dt)
     {
         const T vr0 = face_vel(ma, mb, ra, rb);
-        const T vg1 = (T)(((double)(vr2 + vr0) + 2.0 * (double)vr1) / 4.0);
+        const T vg1 = avg121<T>(vr2 + vr0, vr1);
         const bool up2 = (T)0.5 * vg2 + (T)0.5 * vg1 > (T)0;
         const T bv1 = bw1 * vg1;
         const T bv2 = bw2 * vg2;
         const T f12 = up2 ? bv2 : bv1;
-        const T b12 = (T)((double)bw2 - (double)((f12 - f13) * dt) / 2.0);
+        const T b12 = sub_half<T>(bw2, (f12 - f13) * dt);
         const T g2 = vg2 * b12;
         shift(vr0, vg1, bt, up2, f12, g2);
     }

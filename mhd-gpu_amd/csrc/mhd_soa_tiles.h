@@ -59,9 +59,9 @@ __device__ __forceinline__ T cfl_speed_at(const T *__restrict__ u, const T *__re
     T B;
     if constexpr (AVG) {
         // cell-averaged field (calcfl.c:38-43), C-path arithmetic
-        const T bxa = (T)((double)(b1 + b[sip]) / 2.0);
-        const T bya = (T)((double)(b2 + b[N + sjp]) / 2.0);
-        const T bza = (T)((double)(b3 + b[2 * N + skp]) / 2.0);
+        const T bxa = half_of<T>(b1 + b[sip]);
+        const T bya = half_of<T>(b2 + b[N + sjp]);
+        const T bza = half_of<T>(b3 + b[2 * N + skp]);
         B = bxa * bxa + bya * bya + bza * bza;
     } else {
         B = b1 * b1 + b2 * b2 + b3 * b3;  // face values, calcfl.c:39-50
@@ -99,9 +99,9 @@ __device__ __forceinline__ T cfl_range(const T *__restrict__ u, const T *__restr
                 if constexpr (AVG) {
                     const int ip = i0 + q + 1 == g.nx ? 0 : i0 + q + 1;
                     const T b1p = q + 1 < VE ? vget<T>(Bv[0], q + 1) : b[row + ip];
-                    const T bxa = (T)((double)(b1 + b1p) / 2.0);
-                    const T bya = (T)((double)(b2 + b[N + rowjp + i0 + q]) / 2.0);
-                    const T bza = (T)((double)(b3 + b[2 * N + rowkp + i0 + q]) / 2.0);
+                    const T bxa = half_of<T>(b1 + b1p);
+                    const T bya = half_of<T>(b2 + b[N + rowjp + i0 + q]);
+                    const T bza = half_of<T>(b3 + b[2 * N + rowkp + i0 + q]);
                     B = bxa * bxa + bya * bya + bza * bza;
                 } else {
                     B = b1 * b1 + b2 * b2 + b3 * b3;
@@ -454,9 +454,9 @@ __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__re
                     const T b1p = sb[(w * 3 + 0) * n + ip];
                     const T b2p = w + 1 < ns ? sb[((w + 1) * 3 + 1) * n + i] : sb2n[i];
                     const T b3p = sbk[w * n + i];
-                    const T bxa = (T)((double)(b1 + b1p) / 2.0);
-                    const T bya = (T)((double)(b2 + b2p) / 2.0);
-                    const T bza = (T)((double)(b3 + b3p) / 2.0);
+                    const T bxa = half_of<T>(b1 + b1p);
+                    const T bya = half_of<T>(b2 + b2p);
+                    const T bza = half_of<T>(b3 + b3p);
                     B = bxa * bxa + bya * bya + bza * bza;
                 } else {
                     B = b1 * b1 + b2 * b2 + b3 * b3;
@@ -699,7 +699,7 @@ __device__ __forceinline__ void advect_soa_tile(const T *__restrict__ u, const T
 #pragma unroll
         for (int q = 0; q < K; q++) {
             // A1.c:41 (vx[i-1] + vx[i+1] + 2.0*vx)/4.0
-            vs[q] = (T)(((double)(vxm[q] + vxn[q]) + 2.0 * (double)vx[q]) / 4.0);
+            vs[q] = avg121<T>(vxm[q] + vxn[q], vx[q]);
             const int i = lc * K + q;
             if (it <= ns)
                 bi[q] = it < ns ? sb[(it * 3 + 1) * n + i] : sb2n[i];
@@ -839,7 +839,7 @@ __device__ __forceinline__ void advect_x_tile(const T *__restrict__ u, const T *
         shift_prev<T, K>(vx, vxm, src_prev);
         shift_next<T, K>(vx, vxn, src_next);
 #pragma unroll
-        for (int q = 0; q < K; q++) vs[q] = (T)(((double)(vxm[q] + vxn[q]) + 2.0 * (double)vx[q]) / 4.0);
+        for (int q = 0; q < K; q++) vs[q] = avg121<T>(vxm[q] + vxn[q], vx[q]);
         tvdb_pencil<T, K>(bi, vs, dt, bnew, F, src_next, src_prev);
         if (l < L) {
             store_cells<T, K>(X + it * n + cell0, F);  // 16-byte LDS writes when K allows
@@ -1003,9 +1003,9 @@ __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__rest
             for (int q = 0; q < K; q++) {
                 T B;
                 if constexpr (CFLM == 2) {
-                    const T bxa = (T)((double)(b1[q] + b1p[q]) / 2.0);
-                    const T bya = (T)((double)(b2[q] + b2p[q]) / 2.0);
-                    const T bza = (T)((double)(b3[q] + b3p[q]) / 2.0);
+                    const T bxa = half_of<T>(b1[q] + b1p[q]);
+                    const T bya = half_of<T>(b2[q] + b2p[q]);
+                    const T bza = half_of<T>(b3[q] + b3p[q]);
                     B = bxa * bxa + bya * bya + bza * bza;
                 } else {
                     B = b1[q] * b1[q] + b2[q] * b2[q] + b3[q] * b3[q];
@@ -1148,7 +1148,7 @@ __device__ __forceinline__ void advect_x2_tile(const T *__restrict__ u, const T 
         shift_prev<T, K>(vx, vxm, src_prev);
         shift_next<T, K>(vx, vxn, src_next);
 #pragma unroll
-        for (int q = 0; q < K; q++) vs[q] = (T)(((double)(vxm[q] + vxn[q]) + 2.0 * (double)vx[q]) / 4.0);
+        for (int q = 0; q < K; q++) vs[q] = avg121<T>(vxm[q] + vxn[q], vx[q]);
         tvdb_pencil<T, K>(bi, vs, dt, bnew, F, src_next, src_prev);
         if (l < L) {
             store_cells<T, K>(X + it * n + cell0, F);
