@@ -43,8 +43,9 @@ def profile_value(fname, key):
     if not os.path.exists(path):
         return None
     for line in open(path):
+        line = line.strip()
         if line.startswith(key + ":"):
-            return float(line.split(":")[1])
+            return float(line.split(":")[1].split()[0])
     return None
 
 
@@ -65,6 +66,21 @@ def valu_roofline(fl_ms, n):
     floor_ms = n ** 3 * cyc / (256 * 4) / 2.4e9 * 1e3
     return {"kernel": "fluidx", "bound": "valu", "valu_ops_per_cell": vpc, "simd_cycles_per_cell": cyc,
             "floor_ms": floor_ms, "avg_launch_ms": fl_ms, "frac": floor_ms / fl_ms}
+
+
+def kernels_hbm(sweep_ms, cells):
+    """Every sweep kernel of the step against the HBM peak: algorithmic bytes
+    per launch (each field read once, each updated field written once) over
+    its HIP-event launch time (mhd_ctx_profile_step)."""
+    fam = {"fluidx_x": ([0], FLUIDX_BYTES_PER_CELL), "fluidx_x_with_cfl_and_monitors": ([11], FLUIDX_BYTES_PER_CELL),
+           "fluidx_yz": ([2, 4, 7, 9], FLUIDX_BYTES_PER_CELL), "advect_x": ([1, 10], ADVECT_BYTES_PER_CELL),
+           "advect_yz": ([3, 5, 6, 8], ADVECT_BYTES_PER_CELL)}
+    out = {}
+    for k, (idx, bpc) in fam.items():
+        ms = float(np.mean([sweep_ms[i] for i in idx]))
+        gbs = bpc * cells / (ms * 1e-3) / 1e9
+        out[k] = {"bytes_per_cell": bpc, "avg_launch_ms": ms, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+    return out
 
 
 def host_info():
@@ -341,9 +357,16 @@ def steplevel_report(mhdgpu, n, steps, warmup, state=None):
         step()
     wall = time.perf_counter() - t0
     counts = {"fluidx": 6, "advect": 6, "t12": 2, "t13": 2, "cfl+dt": 1}
+    # algorithmic HBM bytes per cell of each call on the AoS arrays: fluidx
+    # reads u and b and writes u; advectbyzx copies b, then reads u (whole
+    # 20-byte records), b and writes b; a transpose reads u, b and writes
+    # both buffers; the CFL reads u and b
+    bpc = {"fluidx": 52, "advect": 24 + 44, "t12": 32 + 64, "t13": 32 + 64, "cfl+dt": 32}
+    per = {k: v * 1e3 / (steps * counts[k]) for k, v in acc.items()}
     return {"mode": "steplevel", "grid": [n, n, n], "steps": steps, "ms_per_step": wall * 1e3 / steps,
             "zone_updates_per_s": n ** 3 * steps / wall,
-            "ms_per_call": {k: v * 1e3 / (steps * counts[k]) for k, v in acc.items()},
+            "ms_per_call": per,
+            "hbm_GBs_per_call": {k: bpc[k] * n ** 3 / (per[k] * 1e-3) / 1e9 for k in per},
             "note": "synchronous calls; host wall time per call includes the launch and the sync"}
 
 
@@ -463,6 +486,7 @@ def main():
         "step_roofline": {"bytes_per_zone_update": STEP_BYTES_PER_CELL,
                           "achieved_GBs": zus * STEP_BYTES_PER_CELL / 1e9,
                           "frac": zus * STEP_BYTES_PER_CELL / 1e9 / HBM_PEAK_GBS},
+        "kernels_hbm": kernels_hbm(sweep_ms, cells),
         "kernel_ms": {"dt": float(prof[0]), "fluidx_yz_avg": fl_yz_ms, "fluidx_avg_5": fl_ms,
                       "fluidx_x": float(sweep_ms[0]), "fluidx_x_with_cfl_and_monitors": float(sweep_ms[11]),
                       "advect_avg": ad_ms,
