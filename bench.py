@@ -382,10 +382,12 @@ def main():
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-steplevel", action="store_true")
     ap.add_argument("--requests", type=int, default=64)
-    ap.add_argument("--mode", choices=["plain", "copy", "precision", "steplevel"], default="plain",
+    ap.add_argument("--mode", choices=["plain", "copy", "precision", "steplevel", "selfcheck"], default="plain",
                     help="copy: time the plain copy kernel on 1 GiB (counter calibration); "
                          "precision: FP32 vs FP64 report on --ic at --n for --steps; "
-                         "steplevel: the reference's loop through the step-level AoS entry points")
+                         "steplevel: the reference's loop through the step-level AoS entry points; "
+                         "selfcheck: the exhaustive scans of the reciprocal instructions the exact "
+                         "FP32 division and limiter shortcuts rely on")
     ap.add_argument("--ic", choices=["random_phase", "turbulence"], default="random_phase")
     ap.add_argument("--seed", type=int, default=2)
     args = ap.parse_args()
@@ -411,6 +413,16 @@ def main():
 
     if args.mode == "steplevel":
         print(json.dumps(steplevel_report(mhdgpu, args.n, args.steps, args.warmup)), flush=True)
+        return
+
+    if args.mode == "selfcheck":
+        t0 = time.perf_counter()
+        rcp64 = mhdgpu.selfcheck()
+        bad_q, bad_rcp = mhdgpu.selfcheck_division(1)
+        print(json.dumps({"mode": "selfcheck", "rcp_f64_max_rel_err": rcp64,
+                          "quotient_pairs": 1 << 46, "one_correction_quotient_mismatches": bad_q,
+                          "rcp_f32_equivariance_mismatches": bad_rcp,
+                          "wall_s": time.perf_counter() - t0}), flush=True)
         return
 
     if args.mode == "precision":

@@ -55,6 +55,14 @@ int mhd_mem_info(size_t *free_bytes, size_t *total_bytes);
  * instruction over every finite nonzero float-valued D (exhaustive scan,
  * a few milliseconds); the limiter's exactness needs it below 2^-22 */
 int mhd_selfcheck(double *rcp_f64_err);
+/* hardware property the FP32 divisions of the sweeps rely on (no
+ * counterpart in the reference): mismatches[0] counts the pairs a, d in
+ * [1, 2) whose one-correction quotient from the refined v_rcp_f32 differs
+ * from the IEEE a/d, over every dividend and the divisor mantissas 0,
+ * stride, 2*stride, ... (stride 1: all 2^46 pairs, about 30 s);
+ * mismatches[1] counts the floats of exponent [-40, 40] whose v_rcp_f32 is
+ * not exactly the scaled, signed value at the mantissa.  Both must be 0. */
+int mhd_selfcheck_division(int stride, unsigned long long *mismatches);
 
 /* ---- step-level drop-ins (device pointers, FP32) ------------------------
  * Each replaces one wrapper of HIP/hip_subroutine.h:4-12 (CUDA originals in

@@ -219,6 +219,23 @@ int mhd_selfcheck(double *rcp_f64_err)
     return MHD_OK;
 }
 
+int mhd_selfcheck_division(int stride, unsigned long long *mismatches)
+{
+    if (!mismatches || stride < 1) return fail(MHD_ERR_ARG, "mhd_selfcheck_division: bad argument");
+    unsigned long long *d = nullptr;
+    hipError_t e = hipMalloc(&d, 2 * sizeof *d);
+    if (e != hipSuccess) return hip_fail(e, "mhd_selfcheck_division");
+    unsigned long long h[2] = {0, 0};
+    e = hipMemset(d, 0, 2 * sizeof *d);
+    if (e == hipSuccess) e = launch_div_scan((unsigned)stride, d, 0);
+    if (e == hipSuccess) e = hipMemcpy(h, d, sizeof h, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "mhd_selfcheck_division");
+    mismatches[0] = h[0];
+    mismatches[1] = h[1];
+    return MHD_OK;
+}
+
 int mhd_device_sync(void)
 {
     hipError_t e = hipDeviceSynchronize();

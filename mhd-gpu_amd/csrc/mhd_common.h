@@ -240,12 +240,22 @@ __device__ __forceinline__ void mhdflux_cell(const T *u, const T *b, T *v, T &ci
 
 // ---- exact fast FP32 division ------------------------------------------
 // a/d for several a sharing one divisor d: y = rcp(d) refined by one Newton
-// step, then q = a*y and two residual corrections, the same operations the
-// compiler's IEEE division runs (v_div_scale ... v_div_fmas) when its operand
-// scaling does not trigger, i.e. for normal d with |exponent| <= 30 and a of
-// |exponent| <= 60 or a == 0 (the quotient then equals a/d; a zero quotient
-// may differ in sign only).  Callers check those ranges in bulk and fall
-// back to true divisions outside them.
+// step (y = fma(fma(-d, y0, 1), y0, y0)), then q = a*y and ONE residual
+// correction q + fma(-d, q, a)*y.  The compiler's IEEE division runs the
+// same reciprocal and quotient with a second correction (v_div_scale ...
+// v_div_fmas); the first already gives the correctly rounded quotient:
+//  * for a, d in [1, 2) this holds for every one of the 2^46 mantissa pairs
+//    (exhaustive scan on MI355X, mhd_selfcheck_division);
+//  * v_rcp_f32 is exactly odd and exactly scales by powers of two over
+//    divisor exponents [-40, 40] (same scan), and the remaining operations
+//    are IEEE mul/fma whose results stay normal for d of exponent in
+//    [-31, 29] and a of exponent in [-61, 59]: y and q are within 2^90 of
+//    1, and the residual a - d*q is a multiple of 2^(e_a - 47), so at
+//    least 2^-108 when nonzero.  Every such pair is therefore a scanned
+//    pair scaled by powers of two, with the same rounding.
+// A zero a gives a zero quotient that may differ from a/d in sign only.
+// Callers check those ranges in bulk and fall back to true divisions
+// outside them.
 struct Recip {
     float d, y;
 };
@@ -258,11 +268,8 @@ __device__ __forceinline__ Recip recip_of(float d)
 // a/d from the refined reciprocal of d (ranges checked by the caller)
 __device__ __forceinline__ float quot_by(float a, const Recip &r)
 {
-    float q = a * r.y;
-    float e = fmaf(-r.d, q, a);
-    q = fmaf(e, r.y, q);
-    e = fmaf(-r.d, q, a);
-    return fmaf(e, r.y, q);
+    const float q = a * r.y;
+    return fmaf(fmaf(-r.d, q, a), r.y, q);
 }
 
 // mhdflux_cell (FP32) with the four divisions by u1 through one refined

@@ -509,6 +509,61 @@ hipError_t launch_rcp_scan(unsigned long long *maxbits, hipStream_t st)
     return hipGetLastError();
 }
 
+// The shared-divisor quotient quot_by(a, recip_of(d)) against the IEEE
+// division a/d for d = 1 + (d0 + blockIdx.x*stride)*2^-23 and every a in
+// [1, 2): block b takes one divisor, its threads all 2^23 dividends.
+__global__ void __launch_bounds__(256) div_pair_scan_kernel(unsigned d0, unsigned stride,
+                                                            unsigned long long *mismatches)
+{
+    const unsigned dm = d0 + blockIdx.x * stride;
+    const float d = __uint_as_float(0x3f800000u | dm);
+    const Recip r = recip_of(d);
+    unsigned long long bad = 0;
+    for (unsigned am = threadIdx.x; am < (1u << 23); am += 256) {
+        const float a = __uint_as_float(0x3f800000u | am);
+        if (__float_as_uint(quot_by(a, r)) != __float_as_uint(a / d)) bad++;
+    }
+    if (bad) atomicAdd(mismatches, bad);
+}
+
+// v_rcp_f32 of +-m*2^e, m in [1, 2), e in [-40, 40]: +-2^-e times its value
+// at m, bit for bit (the scaling and the sign pass through exactly)
+__global__ void __launch_bounds__(256) rcp_equivariance_kernel(unsigned long long *mismatches)
+{
+    const unsigned long long nt = (unsigned long long)gridDim.x * blockDim.x;
+    unsigned long long bad = 0;
+    for (unsigned long long t = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; t < (81ull << 23);
+         t += nt) {
+        const int e = (int)(t >> 23) - 40;
+        const unsigned m = (unsigned)(t & 0x7fffffu);
+        const float base = __uint_as_float(0x3f800000u | m);
+        const float d = __uint_as_float((unsigned)(127 + e) << 23 | m);
+        const float yd = __builtin_amdgcn_rcpf(d);
+        const float ys = ldexpf(__builtin_amdgcn_rcpf(base), -e);
+        const float yn = __builtin_amdgcn_rcpf(-d);
+        if (__float_as_uint(yd) != __float_as_uint(ys) || __float_as_uint(yn) != (__float_as_uint(yd) ^ 0x80000000u))
+            bad++;
+    }
+    if (bad) atomicAdd(mismatches, bad);
+}
+
+hipError_t launch_div_scan(unsigned stride, unsigned long long *mismatches, hipStream_t st)
+{
+    // divisor mantissas 0, stride, 2*stride, ... < 2^23, at most 2^19 per
+    // launch (about 2 s of work each), each launch finished before the next
+    const unsigned nd = ((1u << 23) + stride - 1) / stride;
+    const unsigned per = 1u << 19;
+    for (unsigned b0 = 0; b0 < nd; b0 += per) {
+        const unsigned nb = min(per, nd - b0);
+        hipLaunchKernelGGL(div_pair_scan_kernel, dim3(nb), dim3(256), 0, st, b0 * stride, stride, mismatches);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(rcp_equivariance_kernel, dim3(4096), dim3(256), 0, st, mismatches + 1);
+    return hipGetLastError();
+}
+
 __global__ void zero_u64_kernel(unsigned long long *p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -906,7 +906,7 @@ __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__rest
                                               T *__restrict__ cfl_part, unsigned long long *cmax = nullptr,
                                               double *__restrict__ mon_part = nullptr)
 {
-    static_assert(P == 1 || (P == 2 || P == 4) && W % P == 0, "pencils per wave: 1, 2 or 4");
+    static_assert(P == 1 || ((P == 2 || P == 4) && W % P == 0), "pencils per wave: 1, 2 or 4");
     constexpr int G = 64 / P;
     const Frame &f = bx.f;
     const TileCtx t = tile_ctx<W>(bx, tile, ntj);

@@ -44,6 +44,7 @@ SIGNATURES = {
     "mhd_dev_copy_kernel": (_I, [_P, _P, _SZ]),
     "mhd_mem_info": (_I, [_SZP, _SZP]),
     "mhd_selfcheck": (_I, [ctypes.POINTER(ctypes.c_double)]),
+    "mhd_selfcheck_division": (_I, [ctypes.c_int, ctypes.POINTER(ctypes.c_ulonglong)]),
     "mhd_fluidx": (_I, [_P, _P, _IP, _IP, _IP, _P]),
     "mhd_advectbyzx": (_I, [_P, _P, _IP, _IP, _IP, _P]),
     "mhd_advectbyzx_noalloc": (_I, [_P, _P, _IP, _IP, _IP, _P]),
@@ -253,6 +254,16 @@ def selfcheck():
     e = ctypes.c_double()
     check(lib().mhd_selfcheck(ctypes.byref(e)))
     return e.value
+
+
+def selfcheck_division(stride=1):
+    """(quotient mismatches, rcp equivariance mismatches) of the FP32
+    shared-divisor division: one-correction quotients against IEEE a/d over
+    every dividend mantissa and the divisor mantissas 0, stride, 2*stride,
+    ...; stride 1 covers all 2^46 pairs.  Both must be 0."""
+    m = (ctypes.c_ulonglong * 2)()
+    check(lib().mhd_selfcheck_division(int(stride), m))
+    return int(m[0]), int(m[1])
 
 
 class Simulation:

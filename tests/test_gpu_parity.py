@@ -46,6 +46,18 @@ def test_rcp_f64_bound_of_the_limiter_quotient(M):
     assert 0.0 < err < 2.0 ** -22
 
 
+def test_shared_divisor_quotient_is_ieee(M):
+    # the FP32 divisions of mhdflux and of the flux normalisation take one
+    # residual correction from the refined reciprocal; that equals the IEEE
+    # quotient for every dividend mantissa against every 61st divisor
+    # mantissa here (all 2^46 pairs with stride 1, bench.py --mode selfcheck),
+    # and the reciprocal instruction scales exactly, which extends it to the
+    # exponent ranges the sweeps admit
+    bad_q, bad_rcp = M.selfcheck_division(61)
+    print("one-correction quotient mismatches %d, rcp equivariance mismatches %d" % (bad_q, bad_rcp), flush=True)
+    assert bad_q == 0 and bad_rcp == 0
+
+
 SHAPES = [(16, 4, 3), (48, 5, 4), (64, 3, 3), (96, 3, 5), (128, 2, 3), (192, 2, 2), (256, 3, 2), (320, 1, 2),
           (384, 2, 1), (512, 1, 2), (24, 1, 1), (40, 7, 1)]
 
