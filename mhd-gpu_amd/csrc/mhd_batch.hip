@@ -1083,7 +1083,10 @@ int Engine::build()
             const BoxDesc &d = boxes_[s];
             const int nt = d.ntj[fr] * d.fr[fr].f.nk;
             kmax_ = std::max(kmax_, d.K[fr]);
-            const int lf = (int)(4 * (9 * kBW + 1) * d.fr[fr].f.ni), la = (int)(4 * (7 * kBW + 2) * d.fr[fr].f.ni);
+            // (the x-frame fluidx with the fused CFL needs fluidx_x_words)
+            const int lf = (int)(4 * std::max((9 * kBW + 1) * d.fr[fr].f.ni,
+                                              fr == 0 ? fluidx_x_words<kBW>(d.fr[fr].f.ni, 1) : 0)),
+                      la = (int)(4 * (7 * kBW + 2) * d.fr[fr].f.ni);
             // schedule 3 runs the x-frame advectbyzx with direct loads (smaller LDS)
             const int lag = fr == 0 ? (int)(4 * advect_x_words<kBW>(d.fr[fr].f.ni)) : la;
             lds_fluidx_ = std::max(lds_fluidx_, lf);

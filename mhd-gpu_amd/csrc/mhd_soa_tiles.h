@@ -898,7 +898,12 @@ __device__ __forceinline__ void advect_x_tile(const T *__restrict__ u, const T *
 // P: pencils per wave (n = 64K/P, LanesGroup); the tile's W pencils then
 // take W/P waves, wave w holding pencils w*P .. w*P+P-1.
 // ---------------------------------------------------------------------------
-template <int W> __host__ __device__ constexpr int fluidx_x_words(int n, int cflm) { return cflm ? 5 * W * n : 0; }
+// LDS of the fused CFL: the new u [W][5][n], the CFL's B^2 per cell [W][n]
+// and each lane's max |div B| (one double per thread)
+template <int W> __host__ __device__ constexpr int fluidx_x_words(int n, int cflm)
+{
+    return cflm ? 6 * W * n + 128 * W : 0;
+}
 
 template <typename T, int K, int W, bool VEC, int CFLM, int P = 1>
 __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__restrict__ b, const SoaBox &bx,
@@ -929,6 +934,9 @@ __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__rest
             for (int q = 0; q < K; q++) o[q] = src[q];
         }
     };
+    T *sB = su + 5 * W * n;
+    double *sD = reinterpret_cast<double *>(su + 6 * W * n);
+    const bool mon = mon_part != nullptr;
     if (ws < ns) {
         const T dt = *dtp;
         T uu[K][5], bxc[K][3];
@@ -954,6 +962,36 @@ __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__rest
                 bxc[q][1] = (T)0.5 * b2[q] + b2p[q] / (T)2;
                 bxc[q][2] = (T)0.5 * b3[q] + b3p[q] / (T)2;
             }
+            if constexpr (CFLM != 0) {
+                // fluidx does not write b: the CFL's B^2 of each cell and
+                // the lane's max |div B| (left faces) of the monitors are
+                // formed now and kept in LDS for after the sweep
+                if (l < L || P > 1) {
+                    T Bc[K];
+#pragma unroll
+                    for (int q = 0; q < K; q++) {
+                        if constexpr (CFLM == 2) {
+                            const T bxa = half_of<T>(b1[q] + b1p[q]);
+                            const T bya = half_of<T>(b2[q] + b2p[q]);
+                            const T bza = half_of<T>(b3[q] + b3p[q]);
+                            Bc[q] = bxa * bxa + bya * bya + bza * bza;
+                        } else {
+                            Bc[q] = b1[q] * b1[q] + b2[q] * b2[q] + b3[q] * b3[q];
+                        }
+                    }
+                    store_cells<T, K>(sB + ws * n + cell0, Bc);
+                    if (mon) {
+                        double dm = 0.0;
+#pragma unroll
+                        for (int q = 0; q < K; q++) {
+                            const double d = ((double)b1p[q] - (double)b1[q]) + ((double)b2p[q] - (double)b2[q]) +
+                                             ((double)b3p[q] - (double)b3[q]);
+                            dm = fmax(dm, fabs(d));
+                        }
+                        sD[threadIdx.x] = dm;
+                    }
+                }
+            }
         }
         if constexpr (P == 1)
             tvd1_pencil<T, K>(uu, bxc, dt, LanesFull{src_next, src_prev});
@@ -977,40 +1015,21 @@ __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__rest
     }
     if constexpr (CFLM != 0) {
         // CFL speed of the new state (calcfl.c:45-56), after a barrier so the
-        // sweep's registers are free: u from LDS, the face fields re-read
-        // (fluidx does not write b); shadow lanes repeat lane l % L
+        // sweep's registers are free: u and the cells' B^2 from LDS; shadow
+        // lanes repeat lane l % L and leave the max unchanged
         __syncthreads();
         T c = (T)0;
         // with mon_part: the FP64 monitors of the new state (mass, energy,
         // momentum x, y, z, max |div B| on the left faces) of the tile's cells
-        const bool mon = mon_part != nullptr;
         double ms[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         if (ws < ns) {
-            T b1[K], b2[K], b3[K];
-            load(b + p, b1);
-            load(b + N + p, b2);
-            load(b + 2 * N + p, b3);
-            T b1p[K], b2p[K], b3p[K];
-            if (CFLM == 2 || mon) {
-                load(b + N + pj, b2p);
-                load(b + 2 * N + pk, b3p);
-                shift_next<T, K>(b1, b1p, src_next);
-            }
-            T um[5][K];
+            T um[5][K], Bc[K];
 #pragma unroll
             for (int m = 0; m < 5; m++) load_cells<T, K>(su + (ws * 5 + m) * n + cell0, um[m]);
+            load_cells<T, K>(sB + ws * n + cell0, Bc);
 #pragma unroll
             for (int q = 0; q < K; q++) {
-                T B;
-                if constexpr (CFLM == 2) {
-                    const T bxa = half_of<T>(b1[q] + b1p[q]);
-                    const T bya = half_of<T>(b2[q] + b2p[q]);
-                    const T bza = half_of<T>(b3[q] + b3p[q]);
-                    B = bxa * bxa + bya * bya + bza * bza;
-                } else {
-                    B = b1[q] * b1[q] + b2[q] * b2[q] + b3[q] * b3[q];
-                }
-                const T sp = cfl_speed<T>(um[0][q], um[1][q], um[2][q], um[3][q], um[4][q], B);
+                const T sp = cfl_speed<T>(um[0][q], um[1][q], um[2][q], um[3][q], um[4][q], Bc[q]);
                 c = cfl_max(c, sp);  // max_2num (calcfl.c:55)
             }
             if (mon && (l < L || P > 1)) {
@@ -1021,10 +1040,8 @@ __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__rest
                     ms[2] += (double)um[1][q];
                     ms[3] += (double)um[2][q];
                     ms[4] += (double)um[3][q];
-                    const double d = ((double)b1p[q] - (double)b1[q]) + ((double)b2p[q] - (double)b2[q]) +
-                                     ((double)b3p[q] - (double)b3[q]);
-                    ms[5] = fmax(ms[5], fabs(d));
                 }
+                ms[5] = sD[threadIdx.x];
             }
         }
         if (mon) monitor_block_store(ms, mon_part + (long)tile * 6);
