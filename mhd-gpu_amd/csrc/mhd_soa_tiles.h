@@ -998,18 +998,27 @@ __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__rest
         else
             tvd1_pencil<T, K>(uu, bxc, dt, LanesGroup<G>{src_next, src_prev});
         if (l < L || P > 1) {
+            // the lane's offsets again, from a copy of the lane index the
+            // compiler cannot see through: it then forms the store addresses
+            // here rather than keeping the load addresses live (and spilled)
+            // across the sweep
+            int lo = l;
+            asm volatile("" : "+v"(lo));
+            const int lcs = P == 1 ? lo % L : lo & (G - 1);
+            const long cs = (long)lcs * K;
+            const long ps = (long)(t.j0 + ws) * f.sj + t.kb + cs;
 #pragma unroll
             for (int m = 0; m < 5; m++) {
                 T o[K];
 #pragma unroll
                 for (int q = 0; q < K; q++) o[q] = uu[q][m];
                 if constexpr (VEC) {
-                    store_cells<T, K>(u + m * N + p, o);
+                    store_cells<T, K>(u + m * N + ps, o);
                 } else {
 #pragma unroll
-                    for (int q = 0; q < K; q++) u[m * N + p + q] = o[q];
+                    for (int q = 0; q < K; q++) u[m * N + ps + q] = o[q];
                 }
-                if constexpr (CFLM != 0) store_cells<T, K>(su + (ws * 5 + m) * n + cell0, o);
+                if constexpr (CFLM != 0) store_cells<T, K>(su + (ws * 5 + m) * n + cs, o);
             }
         }
     }
