@@ -536,6 +536,44 @@ template <typename T, class Speed> __device__ T cfl_after_last_nan(long ncell, c
     return r;
 }
 
+// The block's 6-double monitor record (sums of the first five values, max
+// of the sixth) through LDS in a fixed order, without per-wave shuffles:
+// every thread writes its values to part (6 * blockDim.x doubles, value-
+// major), then thread (v, g) of the first 6 * G threads (G = blockDim.x / 16)
+// folds values g, g + G, ..., g + 15 G of value v, and thread v folds those
+// G partial results.  Deterministic: the order depends on the block shape only.
+// The caller makes part free before (a barrier after its last other use);
+// blockDim.x is a multiple of 64.
+__device__ __forceinline__ void monitor_block_store_lds(const double (&ms)[6], double *dst, double *part)
+{
+    const int T = blockDim.x, t = threadIdx.x, G = T >> 4;
+#pragma unroll
+    for (int v = 0; v < 6; v++) part[v * T + t] = ms[v];
+    __syncthreads();
+    double r = 0.0;
+    if (t < 6 * G) {
+        const int v = t / G, g = t - v * G;
+        const double *pv = part + v * T + g;
+        r = pv[0];
+        if (v < 5) {
+#pragma unroll
+            for (int i = 1; i < 16; i++) r = r + pv[i * G];
+        } else {
+#pragma unroll
+            for (int i = 1; i < 16; i++) r = fmax(r, pv[i * G]);
+        }
+    }
+    __syncthreads();
+    if (t < 6 * G) part[t] = r;
+    __syncthreads();
+    if (t < 6) {
+        const double *pv = part + t * G;
+        double s = pv[0];
+        for (int g = 1; g < G; g++) s = t < 5 ? s + pv[g] : fmax(s, pv[g]);
+        dst[t] = s;
+    }
+}
+
 // FP64 monitors of a block of cells: the lanes' partials ms[0..4] (mass,
 // energy, momentum x, y, z) summed and ms[5] (max |div B|) max-ed, in a
 // fixed order - an xor butterfly in each wave (both lanes of a pair add the

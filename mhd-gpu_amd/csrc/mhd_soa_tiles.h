@@ -899,10 +899,12 @@ __device__ __forceinline__ void advect_x_tile(const T *__restrict__ u, const T *
 // take W/P waves, wave w holding pencils w*P .. w*P+P-1.
 // ---------------------------------------------------------------------------
 // LDS of the fused CFL: the new u [W][5][n], the CFL's B^2 per cell [W][n]
-// and each lane's max |div B| (one double per thread)
+// and each lane's max |div B| (one double per thread); afterwards the same
+// words hold the monitor partials (6 doubles per thread, at most 64 W
+// threads)
 template <int W> __host__ __device__ constexpr int fluidx_x_words(int n, int cflm)
 {
-    return cflm ? 6 * W * n + 128 * W : 0;
+    return cflm ? (6 * W * n + 128 * W > 768 * W ? 6 * W * n + 128 * W : 768 * W) : 0;
 }
 
 template <typename T, int K, int W, bool VEC, int CFLM, int P = 1>
@@ -1053,7 +1055,6 @@ __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__rest
                 ms[5] = sD[threadIdx.x];
             }
         }
-        if (mon) monitor_block_store(ms, mon_part + (long)tile * 6);
         c = wave_cfl_max(c);
         if (lane_id() == 0) cfl_red_slot<T>()[w] = c;
         __syncthreads();
@@ -1068,6 +1069,10 @@ __device__ __forceinline__ void fluidx_x_tile(T *__restrict__ u, const T *__rest
             else
                 cfl_part[tile] = m;
         }
+        // the monitor record through the dynamic LDS, which every wave has
+        // read for the last time before the barrier above (fluidx_x_words
+        // leaves room for 6 doubles per thread)
+        if (mon) monitor_block_store_lds(ms, mon_part + (long)tile * 6, reinterpret_cast<double *>(su));
     }
 }
 
