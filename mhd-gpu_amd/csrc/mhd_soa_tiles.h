@@ -1136,59 +1136,76 @@ __device__ __forceinline__ void advect_x2_tile(const T *__restrict__ u, const T 
     const int src_next = lc + 1 == L ? 0 : lc + 1, src_prev = lc == 0 ? L - 1 : lc - 1;
     const long cell0 = (long)lc * K;
     const T dt = *dtp;
-    for (int it = w; it < nrow; it += nw) {
-        // the item's two pencils (minus side first), its field row and
-        // whether its new field value is this tile's
+    // An item: its two pencils (minus side first), its field row and its
+    // output (the new field value when it is this tile's).  A wave takes the
+    // items w, w + nw, ... and loads the operands of its next item while it
+    // computes the current one, in two register sets that alternate.
+    struct Item {
         long pa, pb;
         const T *fld;
-        T *out = nullptr;
+        T *out;
+        T ma[K], mb[K], ra[K], rb[K], bi[K];
+    };
+    auto fetch = [&](int it, Item &m) {
+        m.out = nullptr;
         if (it < nfa) {  // FA: b2 of pencil j0+a in plane k+p, face velocity from j-1, j
             const int p = it / (ns + 1), a = it - p * (ns + 1);
-            pa = (long)(a == 0 ? jm : j0 + a - 1) * f.sj + kpl[1 + p];
-            pb = (long)(a == ns ? jn : j0 + a) * f.sj + kpl[1 + p];
-            fld = b2i;
-            if (a < ns) out = b2o;
+            m.pa = (long)(a == 0 ? jm : j0 + a - 1) * f.sj + kpl[1 + p];
+            m.pb = (long)(a == ns ? jn : j0 + a) * f.sj + kpl[1 + p];
+            m.fld = b2i;
+            if (a < ns) m.out = b2o;
         } else {  // FB at face k+g: b3 of pencil j0+s in plane k+g, velocity from planes k+g-1, k+g
             const int r = it - nfa, g = r / ns, sp = r - g * ns;
             const long js = (long)(j0 + sp) * f.sj;
-            pa = js + kpl[g];
-            pb = js + kpl[1 + g];
-            fld = b3i;
-            if (g < P) out = b3o;
+            m.pa = js + kpl[g];
+            m.pb = js + kpl[1 + g];
+            m.fld = b3i;
+            if (g < P) m.out = b3o;
         }
-        T ma[K], mb[K], ra[K], rb[K], bi[K];
         if constexpr (VEC) {
-            load_cells<T, K>(mom + pa + cell0, ma);
-            load_cells<T, K>(mom + pb + cell0, mb);
-            load_cells<T, K>(rho + pa + cell0, ra);
-            load_cells<T, K>(rho + pb + cell0, rb);
-            load_cells<T, K>(fld + pb + cell0, bi);
+            load_cells<T, K>(mom + m.pa + cell0, m.ma);
+            load_cells<T, K>(mom + m.pb + cell0, m.mb);
+            load_cells<T, K>(rho + m.pa + cell0, m.ra);
+            load_cells<T, K>(rho + m.pb + cell0, m.rb);
+            load_cells<T, K>(m.fld + m.pb + cell0, m.bi);
         } else {
 #pragma unroll
             for (int q = 0; q < K; q++) {
-                ma[q] = mom[pa + cell0 + q];
-                mb[q] = mom[pb + cell0 + q];
-                ra[q] = rho[pa + cell0 + q];
-                rb[q] = rho[pb + cell0 + q];
-                bi[q] = fld[pb + cell0 + q];
+                m.ma[q] = mom[m.pa + cell0 + q];
+                m.mb[q] = mom[m.pb + cell0 + q];
+                m.ra[q] = rho[m.pa + cell0 + q];
+                m.rb[q] = rho[m.pb + cell0 + q];
+                m.bi[q] = m.fld[m.pb + cell0 + q];
             }
         }
+    };
+    auto compute = [&](int it, const Item &m) {
         T vx[K], vxm[K], vxn[K], vs[K], bnew[K], F[K];
 #pragma unroll
-        for (int q = 0; q < K; q++) vx[q] = face_v(ma[q], mb[q], ra[q], rb[q]);
+        for (int q = 0; q < K; q++) vx[q] = face_v(m.ma[q], m.mb[q], m.ra[q], m.rb[q]);
         shift_prev<T, K>(vx, vxm, src_prev);
         shift_next<T, K>(vx, vxn, src_next);
 #pragma unroll
         for (int q = 0; q < K; q++) vs[q] = avg121<T>(vxm[q] + vxn[q], vx[q]);
-        tvdb_pencil<T, K>(bi, vs, dt, bnew, F, src_next, src_prev);
+        tvdb_pencil<T, K>(m.bi, vs, dt, bnew, F, src_next, src_prev);
         if (l < L) {
             store_cells<T, K>(X + it * n + cell0, F);
-            if (out) {
-                if constexpr (VEC) store_cells<T, K>(out + pb + cell0, bnew);
+            if (m.out) {
+                if constexpr (VEC) store_cells<T, K>(m.out + m.pb + cell0, bnew);
                 else
 #pragma unroll
-                    for (int q = 0; q < K; q++) out[pb + cell0 + q] = bnew[q];
+                    for (int q = 0; q < K; q++) m.out[m.pb + cell0 + q] = bnew[q];
             }
+        }
+    };
+    Item A, B;
+    if (w < nrow) fetch(w, A);
+    for (int it = w; it < nrow; it += 2 * nw) {
+        if (it + nw < nrow) fetch(it + nw, B);
+        compute(it, A);
+        if (it + nw < nrow) {
+            if (it + 2 * nw < nrow) fetch(it + 2 * nw, A);
+            compute(it + nw, B);
         }
     }
     __syncthreads();
