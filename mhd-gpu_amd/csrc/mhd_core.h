@@ -69,9 +69,9 @@ __device__ __forceinline__ void mhdflux_pencil(const T (&uu)[K][5], const T (&bx
     if (c > (T)0) {
         if constexpr (sizeof(T) == 4) {
             // v/c for the 5K fluxes of the lane: c is uniform over the
-            // pencil's lanes, so its reciprocal is formed once and each quotient takes
-            // the remaining five operations of the IEEE division sequence;
-            // that is exact while no operand scaling would trigger (Recip).
+            // pencil's lanes, so its reciprocal is formed once and each
+            // quotient takes a product and one residual correction, which
+            // equals the IEEE quotient over the ranges below (Recip).
             // The operand ranges are checked in bulk: c finite with exponent
             // in [-30, 30]; the largest |v| below 2^60 (which also rules out
             // infinities) and the smallest exponent at least -60 (zero has
