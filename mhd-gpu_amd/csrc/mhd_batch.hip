@@ -1084,7 +1084,7 @@ int Engine::build()
             const int nt = d.ntj[fr] * d.fr[fr].f.nk;
             kmax_ = std::max(kmax_, d.K[fr]);
             // (the x-frame fluidx with the fused CFL needs fluidx_x_words)
-            const int lf = (int)(4 * std::max((9 * kBW + 1) * d.fr[fr].f.ni,
+            const int lf = (int)(4 * std::max(fluidx_soa_words<kBW>(d.fr[fr].f.ni),
                                               fr == 0 ? fluidx_x_words<kBW>(d.fr[fr].f.ni, 1) : 0)),
                       la = (int)(4 * (7 * kBW + 2) * d.fr[fr].f.ni);
             // schedule 3 runs the x-frame advectbyzx with direct loads (smaller LDS)

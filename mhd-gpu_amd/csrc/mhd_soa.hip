@@ -268,7 +268,7 @@ template <typename T, int K, int FR, int P> static hipError_t fluidx_p(T *u, con
     } else {
         constexpr int W2 = 2 * W;
         const int ntj2 = (bx.f.nj + W2 - 1) / W2;
-        const size_t lds2 = sizeof(T) * fluidx_soa_words<W2>(bx.f.ni);
+        const size_t lds2 = sizeof(T) * fluidx_soa_words<W2, T>(bx.f.ni);
         hipLaunchKernelGGL((fluidx_soa_kernel<T, K, W2, FR, 0, P>), dim3(ntj2 * bx.f.nk), dim3(64 * W2 / P), lds2,
                            st, u, b, bx, dt, ntj2);
     }
@@ -283,7 +283,7 @@ static hipError_t fluidx_soa_k(T *u, const T *b, const SoaBox &bx, const T *dt, 
     hipError_t e = soa_attr<T, K, FR>();
     if (e != hipSuccess) return e;
     const int ntj = (bx.f.nj + W - 1) / W;
-    const size_t lds = sizeof(T) * fluidx_soa_words<W>(bx.f.ni);
+    const size_t lds = sizeof(T) * fluidx_soa_words<W, T>(bx.f.ni);
     const dim3 grid(ntj * bx.f.nk), block(64 * W);
     if constexpr (FR == 0) {
         // x frame: direct loads, no LDS staging (LDS only for the fused CFL)
@@ -309,7 +309,7 @@ static hipError_t fluidx_soa_k(T *u, const T *b, const SoaBox &bx, const T *dt, 
         // MHD_YZ_W=1 selects the one-vector tiles.
         static const int yzw = [] { const char *v = getenv("MHD_YZ_W"); return v ? atoi(v) : 2; }();
         constexpr int W2 = 2 * W;
-        const size_t lds2 = sizeof(T) * fluidx_soa_words<W2>(bx.f.ni);
+        const size_t lds2 = sizeof(T) * fluidx_soa_words<W2, T>(bx.f.ni);
         if (yzw >= 2 && lds2 <= 80 * 1024) {
             const int ntj2 = (bx.f.nj + W2 - 1) / W2;
             // a pencil that fills the wave: its length as a constant

@@ -162,7 +162,18 @@ template <int W, int VEW = W> __device__ __forceinline__ TileCtx tile_ctx(const 
 // LDS: su[W][5][n], sb[W][3][n], sb2n[n] (b2 of the pencil after the tile),
 //      sbk[W][n] (b3 of the tile's pencils at k+1)
 // ---------------------------------------------------------------------------
-template <int W> __host__ __device__ constexpr int fluidx_soa_words(int n) { return (9 * W + 1) * n; }
+// Slot strides of the tile's LDS arrays: each slot (pencil) is padded by 16
+// bytes, so that the two halves of a tile row, four slots apart, fall in
+// different LDS banks when a wave writes them (a row's vector pair) or
+// reads them back (the store phase).
+template <typename T> __host__ __device__ constexpr int slot_pad() { return 16 / (int)sizeof(T); }
+template <typename T> __host__ __device__ constexpr int su_slot(int n) { return 5 * n + slot_pad<T>(); }
+template <typename T> __host__ __device__ constexpr int sb_slot(int n) { return 3 * n + slot_pad<T>(); }
+template <typename T> __host__ __device__ constexpr int sk_slot(int n) { return n + slot_pad<T>(); }
+template <int W, typename T = float> __host__ __device__ constexpr int fluidx_soa_words(int n)
+{
+    return W * (su_slot<T>(n) + sb_slot<T>(n) + sk_slot<T>(n)) + n;
+}
 
 // K consecutive cells from / to p, as 16-byte vectors when K allows (p
 // then 16-byte aligned)
@@ -234,10 +245,10 @@ __device__ __forceinline__ void yz_row_scatter(const YzRow<T> &r, T *su, T *sb, 
     for (int v = 0; v < VE; v++) {
         const int s = h * VE + v;
 #pragma unroll
-        for (int c = 0; c < 5; c++) su[(s * 5 + c) * n + i] = vget<T>(r.u[c], v);
+        for (int c = 0; c < 5; c++) su[s * su_slot<T>(n) + c * n + i] = vget<T>(r.u[c], v);
 #pragma unroll
-        for (int c = 0; c < 3; c++) sb[(s * 3 + c) * n + i] = vget<T>(r.b[c], v);
-        sbk[s * n + i] = vget<T>(r.b[3], v);
+        for (int c = 0; c < 3; c++) sb[s * sb_slot<T>(n) + c * n + i] = vget<T>(r.b[c], v);
+        sbk[s * sk_slot<T>(n) + i] = vget<T>(r.b[3], v);
     }
     if (last) sb2n[i] = r.b2n;
 }
@@ -264,17 +275,17 @@ __device__ __forceinline__ void fluidx_lds_pencils(T *su, const T *sb, const T *
 #pragma unroll
         for (int m = 0; m < 5; m++) {
             T x[K];
-            load_cells<T, K>(su + (w * 5 + m) * n + lc * K, x);
+            load_cells<T, K>(su + w * su_slot<T>(n) + m * n + lc * K, x);
 #pragma unroll
             for (int q = 0; q < K; q++) uu[q][m] = x[q];
         }
         {
             T b1[K], b1p[K], b2[K], b2p[K], b3[K], b3p[K];
-            load_cells<T, K>(sb + (w * 3 + 0) * n + lc * K, b1);
-            load_cells<T, K>(sb + (w * 3 + 1) * n + lc * K, b2);
-            load_cells<T, K>((w + 1 < ns ? sb + ((w + 1) * 3 + 1) * n : sb2n) + lc * K, b2p);
-            load_cells<T, K>(sb + (w * 3 + 2) * n + lc * K, b3);
-            load_cells<T, K>(sbk + w * n + lc * K, b3p);
+            load_cells<T, K>(sb + w * sb_slot<T>(n) + lc * K, b1);
+            load_cells<T, K>(sb + w * sb_slot<T>(n) + n + lc * K, b2);
+            load_cells<T, K>((w + 1 < ns ? sb + (w + 1) * sb_slot<T>(n) + n : sb2n) + lc * K, b2p);
+            load_cells<T, K>(sb + w * sb_slot<T>(n) + 2 * n + lc * K, b3);
+            load_cells<T, K>(sbk + w * sk_slot<T>(n) + lc * K, b3p);
             shift_next<T, K>(b1, b1p, src_next);  // b1 at i+1, periodic along the pencil
 #pragma unroll
             for (int q = 0; q < K; q++) {
@@ -294,7 +305,7 @@ __device__ __forceinline__ void fluidx_lds_pencils(T *su, const T *sb, const T *
                 T x[K];
 #pragma unroll
                 for (int q = 0; q < K; q++) x[q] = uu[q][m];
-                store_cells<T, K>(su + (w * 5 + m) * n + lc * K, x);
+                store_cells<T, K>(su + w * su_slot<T>(n) + m * n + lc * K, x);
             }
         }
     }
@@ -316,12 +327,13 @@ __device__ __forceinline__ void fluidx_yz_store(T *__restrict__ u, const T *su, 
         T *uc = u + cu<FR>(c) * N + t.j0 + t.kb;
         for (int e = tid; e < n * H; e += nt) {
             const int i = H == 2 ? e >> 1 : e, h = e - i * H;
-            const T *sv = su + (h * VE * 5 + c) * n + i;
+            const int us = su_slot<T>(n);
+            const T *sv = su + h * VE * us + c * n + i;
             V16<T> v;
             if constexpr (sizeof(T) == 4)
-                v = make_float4(sv[0], sv[5 * n], sv[10 * n], sv[15 * n]);
+                v = make_float4(sv[0], sv[us], sv[2 * us], sv[3 * us]);
             else
-                v = make_double2(sv[0], sv[5 * n]);
+                v = make_double2(sv[0], sv[us]);
             *reinterpret_cast<V16<T> *>(uc + (long)i * bx.f.si + h * VE) = v;
         }
     }
@@ -347,7 +359,8 @@ __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__re
     const TileCtx t = tile_ctx<W, VE>(bx, tile, ntj);
     const int n = NF ? NF : t.n, ns = t.ns, L = n / K;
     const long N = bx.N;
-    T *su = smem, *sb = su + 5 * W * n, *sb2n = sb + 3 * W * n, *sbk = sb2n + n;
+    const int us = su_slot<T>(n), bs = sb_slot<T>(n), ks = sk_slot<T>(n);
+    T *su = smem, *sb = su + W * us, *sb2n = sb + W * bs, *sbk = sb2n + n;
     const int jn = wrapn(t.j0 + ns, f.nj);  // pencil after the tile (jp of the last one)
     const int tid = threadIdx.x, nt = blockDim.x;
 
@@ -359,19 +372,19 @@ __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__re
         auto item = [&](int e, int &doff) -> const T * {
             if (e < eU) {
                 const int sc = fdiv(e, bx.n4_div), i4 = e - sc * n4, s = sc / 5, c = sc - 5 * s;
-                doff = (s * 5 + c) * n + i4 * VE;
+                doff = s * us + c * n + i4 * VE;
                 return u + cu<FR>(c) * N + (long)(t.j0 + s) * f.sj + t.kb + i4 * VE;
             } else if (e < eB) {
                 const int r = e - eU, sc = fdiv(r, bx.n4_div), i4 = r - sc * n4, s = sc / 3, c = sc - 3 * s;
-                doff = 5 * W * n + (s * 3 + c) * n + i4 * VE;
+                doff = W * us + s * bs + c * n + i4 * VE;
                 return b + cb<FR>(c) * N + (long)(t.j0 + s) * f.sj + t.kb + i4 * VE;
             } else if (e < eN) {
                 const int i4 = e - eB;
-                doff = 8 * W * n + i4 * VE;
+                doff = W * (us + bs) + i4 * VE;
                 return b + Comp<FR>::b[1] * N + (long)jn * f.sj + t.kb + i4 * VE;
             } else {
                 const int r = e - eN, s = fdiv(r, bx.n4_div), i4 = r - s * n4;
-                doff = 8 * W * n + n + s * n + i4 * VE;
+                doff = W * (us + bs) + n + s * ks + i4 * VE;
                 return b + Comp<FR>::b[2] * N + (long)(t.j0 + s) * f.sj + t.kpb + i4 * VE;
             }
         };
@@ -412,14 +425,14 @@ __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__re
                 i = e - r * n;
                 s = r / 5;
                 c = r - 5 * s;
-                su[(s * 5 + c) * n + i] =
+                su[s * us + c * n + i] =
                     u[cu<FR>(c) * N + t.kb + (long)i * f.si + (long)(t.j0 + s) * f.sj];
             } else if (e < eB) {
                 const int q = e - eU, r = fdiv(q, bx.n_div);
                 i = q - r * n;
                 s = r / 3;
                 c = r - 3 * s;
-                sb[(s * 3 + c) * n + i] =
+                sb[s * bs + c * n + i] =
                     b[cb<FR>(c) * N + t.kb + (long)i * f.si + (long)(t.j0 + s) * f.sj];
             } else if (e < eN) {
                 i = e - eB;
@@ -428,7 +441,7 @@ __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__re
                 const int q = e - eN;
                 s = fdiv(q, bx.n_div);
                 i = q - s * n;
-                sbk[s * n + i] = b[Comp<FR>::b[2] * N + t.kpb + (long)i * f.si + (long)(t.j0 + s) * f.sj];
+                sbk[s * ks + i] = b[Comp<FR>::b[2] * N + t.kpb + (long)i * f.si + (long)(t.j0 + s) * f.sj];
             }
         }
     }
@@ -446,14 +459,14 @@ __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__re
 #pragma unroll
             for (int q = 0; q < K; q++) {
                 const int i = lc * K + q;
-                const T *ui = su + w * 5 * n + i;
-                const T b1 = sb[(w * 3 + 0) * n + i], b2 = sb[(w * 3 + 1) * n + i], b3 = sb[(w * 3 + 2) * n + i];
+                const T *ui = su + w * us + i;
+                const T b1 = sb[w * bs + i], b2 = sb[w * bs + n + i], b3 = sb[w * bs + 2 * n + i];
                 T B;
                 if constexpr (CFLM == 2) {
                     const int ip = i + 1 == n ? 0 : i + 1;
-                    const T b1p = sb[(w * 3 + 0) * n + ip];
-                    const T b2p = w + 1 < ns ? sb[((w + 1) * 3 + 1) * n + i] : sb2n[i];
-                    const T b3p = sbk[w * n + i];
+                    const T b1p = sb[w * bs + ip];
+                    const T b2p = w + 1 < ns ? sb[(w + 1) * bs + n + i] : sb2n[i];
+                    const T b3p = sbk[w * ks + i];
                     const T bxa = half_of<T>(b1 + b1p);
                     const T bya = half_of<T>(b2 + b2p);
                     const T bza = half_of<T>(b3 + b3p);
@@ -481,14 +494,14 @@ __device__ __forceinline__ void fluidx_soa_tile(T *__restrict__ u, const T *__re
         for (int e = tid; e < eU; e += nt) {
             const int sc = fdiv(e, bx.n4_div), i4 = e - sc * n4, s = sc / 5, c = sc - 5 * s;
             *reinterpret_cast<V16<T> *>(u + cu<FR>(c) * N + (long)(t.j0 + s) * f.sj + t.kb + i4 * VE) =
-                *reinterpret_cast<const V16<T> *>(su + (s * 5 + c) * n + i4 * VE);
+                *reinterpret_cast<const V16<T> *>(su + s * us + c * n + i4 * VE);
         }
     } else if (FR != 0 && t.vec) {
         fluidx_yz_store<T, FR>(u, su, bx, t, n);
     } else {
         for (int e = tid; e < ns * 5 * n; e += nt) {
             const int r = fdiv(e, bx.n_div), i = e - r * n, s = r / 5, c = r - 5 * s;
-            u[cu<FR>(c) * N + t.kb + (long)i * f.si + (long)(t.j0 + s) * f.sj] = su[(s * 5 + c) * n + i];
+            u[cu<FR>(c) * N + t.kb + (long)i * f.si + (long)(t.j0 + s) * f.sj] = su[s * us + c * n + i];
         }
     }
 }
