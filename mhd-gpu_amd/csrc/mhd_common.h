@@ -95,24 +95,38 @@ template <typename T> __device__ __forceinline__ T gamma_c() { return (T)(5.0 / 
 //    (N has 48 significant bits, D 24), so q' and the IEEE quotient round
 //    to the same float.
 // The ranges involved (|N| in [2^-297, 2^257], |D| in [2^-149, 2^128])
-// keep every step clear of double overflow and underflow; an infinite D
-// (p + m overflowed) takes the IEEE division.
+// keep every step clear of double overflow and underflow.
+__device__ __forceinline__ double limiter_refined(double N, double D)
+{
+    double y = __builtin_amdgcn_rcp(D);
+    const double e = fma(-D, y, 1.0);
+    y = fma(y, e, y);
+    const double q = N * y;
+    const double r = fma(-D, q, N);
+    return fma(r, y, q);
+}
+// An infinite D (p + m overflowed) takes the IEEE division.
 __device__ __forceinline__ float limiter_quotient(float p, float m)
 {
     const float s = p + m;
     const double N = 2.0 * (double)p * (double)m, D = (double)s;
     double q;
-    if (__builtin_isfinite(s)) {
-        double y = __builtin_amdgcn_rcp(D);
-        const double e = fma(-D, y, 1.0);
-        y = fma(y, e, y);
-        q = N * y;
-        const double r = fma(-D, q, N);
-        q = fma(r, y, q);
-    } else {
+    if (__builtin_isfinite(s))
+        q = limiter_refined(N, D);
+    else
         q = N / D;
-    }
     return (float)q;
+}
+// The same without a branch, for p*m > 0: an infinite sum (p + m
+// overflowed in FP32, or p or m infinite) enters as D = +-2^1000.  For
+// finite N the quotient is then below 2^-740 and rounds to the float zero
+// of the sign of D, which is N/(+-inf); for N = +inf (an infinite operand)
+// q is infinite, the remainder NaN, and so is the result, as inf/inf.
+__device__ __forceinline__ float limiter_quotient_nb(float p, float m)
+{
+    const float s = p + m;
+    const double N = 2.0 * (double)p * (double)m;
+    return (float)limiter_refined(N, fmin(fmax((double)s, -0x1p1000), 0x1p1000));
 }
 
 // IEEE FP64 square root for x = 0, +inf or x >= 2^-767: the compiler's own
@@ -154,6 +168,17 @@ template <typename T> __device__ __forceinline__ T vanleer(T p, T m)
             d = (T)(2.0 * (double)p * (double)m / (double)(p + m));
     }
     return d;
+}
+// The same with the FP32 quotient formed on every lane and selected where
+// p*m > 0: no branch (the streamed tvdb, whose waves run it once per row)
+template <typename T> __device__ __forceinline__ T vanleer_nb(T p, T m)
+{
+    if constexpr (sizeof(T) == 4) {
+        const float d = limiter_quotient_nb(p, m);
+        return p * m > 0.0f ? d : 0.0f;
+    } else {
+        return vanleer<T>(p, m);
+    }
 }
 
 // Exact shortcuts of three FP64 sub-expressions of the reference's FP32 code

@@ -67,7 +67,7 @@ template <typename T> struct TvdbStream {
             wp = half_of<T>(w - g2);
             wm = half_of<T>(g4 - w);
         }
-        const T dw = vanleer<T>(wm, wp);
+        const T dw = vanleer_nb<T>(wm, wp);
         const T fl4 = (w + dw) * dt;
         // tvdb.c:87 b = b - (flux - flux[i-1]) at t-4
         bnew = bw4 - (fl4 - fl5);

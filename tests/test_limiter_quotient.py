@@ -14,6 +14,8 @@ import numpy as np
 
 
 def fma(a, b, c):
+    if not all(np.isfinite(x) for x in (a, b, c)):
+        return a * b + c  # infinities and NaN: no rounding to model
     return float(Fraction(a) * Fraction(b) + Fraction(c))
 
 
@@ -25,10 +27,13 @@ def ieee(p, m):
     return np.float32(2.0 * float(p) * float(m) / float(s))
 
 
-def fast_double(p, m, delta):
-    """the double quotient before the final rounding to FP32"""
+def fast_double(p, m, delta, clamp=False):
+    """the double quotient before the final rounding to FP32; clamp: the
+    branch-free form, an infinite p + m entering as +-2^1000"""
     s = np.float32(p + m)
     N, D = 2.0 * float(p) * float(m), float(s)
+    if clamp:
+        D = min(max(D, -2.0 ** 1000), 2.0 ** 1000)
     y = float(Fraction(1) / Fraction(D) * (1 + delta))  # planted estimate error
     e = fma(-D, y, 1.0)
     y = fma(y, e, y)
@@ -37,11 +42,15 @@ def fast_double(p, m, delta):
     return fma(r, y, q)
 
 
-def fast(p, m, delta):
+def fast(p, m, delta, clamp=False):
     p, m = np.float32(p), np.float32(m)
     if not np.float32(p * m) > 0:
         return np.float32(0)
-    return np.float32(fast_double(p, m, delta))
+    return np.float32(fast_double(p, m, delta, clamp))
+
+
+def same32(a, b):
+    return a.view(np.uint32) == b.view(np.uint32) or (np.isnan(a) and np.isnan(b))
 
 
 def test_limiter_quotient_equals_ieee_with_planted_reciprocal_error():
@@ -82,3 +91,31 @@ def test_limiter_quotient_exact_quotients():
         for p, m in cases:
             for d in (eps, -eps, Fraction(0)):
                 assert ieee(p, m).view(np.uint32) == fast(p, m, d).view(np.uint32), (p, m, d)
+
+
+def test_branch_free_quotient_on_overflowing_sums():
+    """the branch-free form (streamed tvdb): every pair whose FP32 sum
+    overflows or holds an infinite operand gives the IEEE result, a signed
+    zero or NaN; a clamp bound small enough to leave a nonzero float (2^200)
+    fails"""
+    big = [np.float32(x) for x in (2.0 ** 127, 3.0e38, 1.7e38, np.finfo(np.float32).max)]
+    rng = np.random.default_rng(5)
+    pairs = [(a, b) for a in big for b in big]
+    pairs += [(np.float32(2.0 ** 127 * (1 + rng.random())), np.float32(2.0 ** 127 * (1 + rng.random())))
+              for _ in range(200)]
+    pairs += [(np.float32(np.inf), np.float32(x)) for x in (1.0, 2.0 ** -149, 3.0e38)]
+    pairs += [(np.float32(np.inf), np.float32(np.inf))]
+    pairs += [(-a, -b) for a, b in pairs]
+    eps = Fraction(1, 1 << 24)
+    with np.errstate(all="ignore"):
+        pairs = [(p, m) for p, m in pairs if not np.isfinite(np.float32(p + m))]
+        assert len(pairs) > 300
+        for p, m in pairs:
+            want = ieee(p, m)
+            for d in (eps, -eps, Fraction(0)):
+                assert same32(want, fast(p, m, d, clamp=True)), (p, m, d)
+        # a bound that is too small leaves 2pm/D representable: the check
+        # above is not vacuous
+        p, m = big[0], big[0]
+        N, D = 2.0 * float(p) * float(m), 2.0 ** 200
+        assert np.float32(N / D) != 0
