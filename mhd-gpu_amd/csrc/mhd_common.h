@@ -186,8 +186,13 @@ template <typename T> __device__ __forceinline__ T vanleer_nb(T p, T m)
 //  * (float)((double)x / 2.0): the halving is exact in double, so this is
 //    x/2 rounded once to float, which the FP32 product x*0.5f also is
 //    (FP32 denormals are kept): one operation instead of three;
-//  * (float)((double)a - (double)x / 2.0): x/2 is exact, so the difference
-//    is one fma;
+//  * (float)((double)a - (double)x / 2.0): x/2 is exact, so this is the
+//    difference of two numbers of 24 significant bits rounded to double and
+//    then to float.  That double rounding is innocuous: where a - x/2 is not
+//    a double, the exponents differ by 29 or more and the result is within
+//    1/32 of an ulp of the larger operand, which both roundings return.  So
+//    it is a - x/2 rounded once to float, the FP32 fma(x, -0.5, a): one
+//    operation instead of four;
 //  * (float)(((double)s + 2.0*(double)v) / 4.0): 2v is exact, so the sum
 //    is one fma, and /4 is exact.
 // The FP64 instantiation keeps the literal expressions.
@@ -201,7 +206,7 @@ template <typename T> __device__ __forceinline__ T half_of(T x)
 template <typename T> __device__ __forceinline__ T sub_half(T a, T x)
 {
     if constexpr (sizeof(T) == 4)
-        return (T)fma((double)x, -0.5, (double)a);
+        return fmaf(x, -0.5f, a);
     else
         return (T)((double)a - (double)x / 2.0);
 }
@@ -232,12 +237,12 @@ template <typename T> __device__ __forceinline__ T pstar_q(T q, T u5, T B)
     }
 }
 template <typename T> __device__ __forceinline__ T pstar(T u1, T u5, T S, T B) { return pstar_q<T>(S / u1, u5, B); }
-// p = ps - B/2.0 in double (mhdflux.c:37, calcfl.c:52); for FP32 B/2.0 is
-// exact, so the difference is one fma
+// p = ps - B/2.0 in double (mhdflux.c:37, calcfl.c:52); for FP32 this is
+// ps - B/2 rounded once to float, as sub_half: one FP32 fma
 template <typename T> __device__ __forceinline__ T pgas(T ps, T B)
 {
     if constexpr (sizeof(T) == 4)
-        return (T)fma((double)B, -0.5, (double)ps);
+        return fmaf(B, -0.5f, ps);
     else
         return (T)((double)ps - (double)B / 2.0);
 }
