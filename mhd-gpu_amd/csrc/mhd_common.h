@@ -105,11 +105,19 @@ __device__ __forceinline__ double limiter_refined(double N, double D)
     const double r = fma(-D, q, N);
     return fma(r, y, q);
 }
+// N = 2.0*(double)p*(double)m for FP32 p, m: exact (a 48-bit product).
+__device__ __forceinline__ double limiter_numerator(float p, float m) { return 2.0 * (double)p * (double)m; }
+// The limiter's test p*m > 0 (in FP32) from N: the FP32 product rounds to a
+// positive value exactly when p*m > 2^-150 (2^-150 itself is the tie
+// between 0 and the smallest subnormal, which goes to 0), infinities and
+// FP32 overflow included, and NaN fails both.  One compare on N, which the
+// quotient needs anyway, instead of an FP32 product and a compare.
+__device__ __forceinline__ bool limiter_taken(double N) { return N > 0x1p-149; }
 // An infinite D (p + m overflowed) takes the IEEE division.
-__device__ __forceinline__ float limiter_quotient(float p, float m)
+__device__ __forceinline__ float limiter_quotient(float p, float m, double N)
 {
     const float s = p + m;
-    const double N = 2.0 * (double)p * (double)m, D = (double)s;
+    const double D = (double)s;
     double q;
     if (__builtin_isfinite(s))
         q = limiter_refined(N, D);
@@ -122,10 +130,9 @@ __device__ __forceinline__ float limiter_quotient(float p, float m)
 // finite N the quotient is then below 2^-740 and rounds to the float zero
 // of the sign of D, which is N/(+-inf); for N = +inf (an infinite operand)
 // q is infinite, the remainder NaN, and so is the result, as inf/inf.
-__device__ __forceinline__ float limiter_quotient_nb(float p, float m)
+__device__ __forceinline__ float limiter_quotient_nb(float p, float m, double N)
 {
     const float s = p + m;
-    const double N = 2.0 * (double)p * (double)m;
     return (float)limiter_refined(N, fmin(fmax((double)s, -0x1p1000), 0x1p1000));
 }
 
@@ -161,11 +168,11 @@ template <typename T> __device__ __forceinline__ double sqrt_of(double x)
 template <typename T> __device__ __forceinline__ T vanleer(T p, T m)
 {
     T d = (T)0;
-    if (p * m > (T)0) {
-        if constexpr (sizeof(T) == 4)
-            d = limiter_quotient(p, m);
-        else
-            d = (T)(2.0 * (double)p * (double)m / (double)(p + m));
+    if constexpr (sizeof(T) == 4) {
+        const double N = limiter_numerator(p, m);
+        if (limiter_taken(N)) d = limiter_quotient(p, m, N);
+    } else {
+        if (p * m > (T)0) d = (T)(2.0 * (double)p * (double)m / (double)(p + m));
     }
     return d;
 }
@@ -174,8 +181,9 @@ template <typename T> __device__ __forceinline__ T vanleer(T p, T m)
 template <typename T> __device__ __forceinline__ T vanleer_nb(T p, T m)
 {
     if constexpr (sizeof(T) == 4) {
-        const float d = limiter_quotient_nb(p, m);
-        return p * m > 0.0f ? d : 0.0f;
+        const double N = limiter_numerator(p, m);
+        const float d = limiter_quotient_nb(p, m, N);
+        return limiter_taken(N) ? d : 0.0f;
     } else {
         return vanleer<T>(p, m);
     }

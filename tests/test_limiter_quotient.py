@@ -119,3 +119,44 @@ def test_branch_free_quotient_on_overflowing_sums():
         p, m = big[0], big[0]
         N, D = 2.0 * float(p) * float(m), 2.0 ** 200
         assert np.float32(N / D) != 0
+
+
+def test_limiter_test_from_the_fp64_numerator():
+    """p*m > 0 in FP32 (tvd1.c:71, tvdb.c:72) equals 2.0*(double)p*(double)m
+    > 2^-149 (the sweeps form that exact product anyway): the FP32 product is
+    positive exactly when the real one exceeds 2^-150, the tie at 2^-150
+    rounding to zero.  Random bit patterns, products around the underflow
+    threshold (both sides of 2^-150 and the tie itself), FP32 overflow,
+    infinities, zeros and NaN.  A threshold 2^-30 off either way fails."""
+    rng = np.random.default_rng(17)
+    bits = rng.integers(0, 1 << 32, size=(20000, 2), dtype=np.uint64).astype(np.uint32)
+    pairs = [(bits[i, 0].view(np.float32), bits[i, 1].view(np.float32)) for i in range(len(bits))]
+    for _ in range(4000):
+        e = int(rng.integers(-149, 0))
+        p = np.float32(np.ldexp(1 + rng.random(), e))
+        m = np.float32(np.ldexp(1 + rng.random(), -150 - e + int(rng.integers(-2, 2))))
+        pairs.append((p, m))
+    tie = [(np.float32(2.0 ** -75), np.float32(2.0 ** -75)), (np.float32(2.0 ** -149), np.float32(0.5)),
+           (np.float32(2.0 ** -149), np.float32(0.75)), (np.float32(2.0 ** -149), np.float32(0.5000001)),
+           (np.float32(3.0e38), np.float32(2.0)), (np.float32(np.inf), np.float32(1.0)),
+           (np.float32(np.inf), np.float32(0.0)), (np.float32(-0.0), np.float32(-0.0)),
+           (np.float32(np.nan), np.float32(1.0)), (np.float32(-np.inf), np.float32(-2.0 ** -149))]
+    # just above the tie: for each p the smallest m with p*m > 2^-150, so that
+    # the product exceeds 2^-150 by as little as two floats allow
+    for x in rng.random(3000):
+        p = np.float32(np.ldexp(1 + x, -75))
+        m = np.float32(2.0 ** -150 / float(p))
+        while float(p) * float(m) <= 2.0 ** -150:
+            m = np.nextafter(m, np.float32(1))
+        tie.append((p, m))
+    pairs += tie
+    lo, hi = 2.0 ** -149 * (1 - 2.0 ** -30), 2.0 ** -149 * (1 + 2.0 ** -30)
+    miss_lo = miss_hi = 0
+    with np.errstate(all="ignore"):
+        for p, m in pairs:
+            want = bool(np.float32(p) * np.float32(m) > 0)
+            N = 2.0 * float(p) * float(m)
+            assert (N > 2.0 ** -149) == want, (p, m)
+            miss_lo += (N > lo) != want
+            miss_hi += (N > hi) != want
+    assert miss_lo > 0 and miss_hi > 0
