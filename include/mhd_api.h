@@ -63,6 +63,13 @@ int mhd_selfcheck(double *rcp_f64_err);
  * mismatches[1] counts the floats of exponent [-40, 40] whose v_rcp_f32 is
  * not exactly the scaled, signed value at the mantissa.  Both must be 0. */
 int mhd_selfcheck_division(int stride, unsigned long long *mismatches);
+/* the FP32 van Leer limiter as the sweeps evaluate it (no counterpart in the
+ * reference; a check of the limiter shortcuts on the device): out[i] from
+ * the branching form of the tiles, out_nb[i] from the branch-free form of the
+ * streamed tvdb, for the operand pairs (p[i], m[i]); host arrays of n floats.
+ * Both must equal (float)(2.0*p*m/(p+m)) where the FP32 p*m > 0 and 0
+ * elsewhere (tvd1.c:71-81, tvdb.c:72-76). */
+int mhd_selfcheck_limiter(const float *p, const float *m, float *out, float *out_nb, int n);
 
 /* ---- step-level drop-ins (device pointers, FP32) ------------------------
  * Each replaces one wrapper of HIP/hip_subroutine.h:4-12 (CUDA originals in

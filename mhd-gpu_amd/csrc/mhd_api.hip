@@ -236,6 +236,24 @@ int mhd_selfcheck_division(int stride, unsigned long long *mismatches)
     return MHD_OK;
 }
 
+int mhd_selfcheck_limiter(const float *p, const float *m, float *out, float *out_nb, int n)
+{
+    if (!p || !m || !out || !out_nb || n < 0) return fail(MHD_ERR_ARG, "mhd_selfcheck_limiter: bad argument");
+    if (n == 0) return MHD_OK;
+    const size_t bytes = (size_t)n * sizeof(float);
+    float *d = nullptr;
+    hipError_t e = hipMalloc(&d, 4 * bytes);
+    if (e != hipSuccess) return hip_fail(e, "mhd_selfcheck_limiter");
+    e = hipMemcpy(d, p, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + n, m, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_limiter_eval(d, d + n, d + 2 * (size_t)n, d + 3 * (size_t)n, n, 0);
+    if (e == hipSuccess) e = hipMemcpy(out, d + 2 * (size_t)n, bytes, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(out_nb, d + 3 * (size_t)n, bytes, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "mhd_selfcheck_limiter");
+    return MHD_OK;
+}
+
 int mhd_device_sync(void)
 {
     hipError_t e = hipDeviceSynchronize();

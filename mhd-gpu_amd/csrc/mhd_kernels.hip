@@ -564,6 +564,26 @@ hipError_t launch_div_scan(unsigned stride, unsigned long long *mismatches, hipS
     return hipGetLastError();
 }
 
+// Both FP32 forms of the van Leer limiter (tvd1.c:71-81, tvdb.c:72-76) on
+// given operand pairs: the branching one of the tiles and the branch-free
+// one of the streamed tvdb
+__global__ void __launch_bounds__(256) limiter_eval_kernel(const float *__restrict__ p, const float *__restrict__ m,
+                                                           float *__restrict__ out, float *__restrict__ out_nb, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        out[i] = vanleer<float>(p[i], m[i]);
+        out_nb[i] = vanleer_nb<float>(p[i], m[i]);
+    }
+}
+
+hipError_t launch_limiter_eval(const float *p, const float *m, float *out, float *out_nb, int n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(limiter_eval_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, m, out, out_nb, n);
+    return hipGetLastError();
+}
+
 __global__ void zero_u64_kernel(unsigned long long *p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;

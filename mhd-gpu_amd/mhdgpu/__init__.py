@@ -45,6 +45,7 @@ SIGNATURES = {
     "mhd_mem_info": (_I, [_SZP, _SZP]),
     "mhd_selfcheck": (_I, [ctypes.POINTER(ctypes.c_double)]),
     "mhd_selfcheck_division": (_I, [ctypes.c_int, ctypes.POINTER(ctypes.c_ulonglong)]),
+    "mhd_selfcheck_limiter": (_I, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
     "mhd_fluidx": (_I, [_P, _P, _IP, _IP, _IP, _P]),
     "mhd_advectbyzx": (_I, [_P, _P, _IP, _IP, _IP, _P]),
     "mhd_advectbyzx_noalloc": (_I, [_P, _P, _IP, _IP, _IP, _P]),
@@ -264,6 +265,19 @@ def selfcheck_division(stride=1):
     m = (ctypes.c_ulonglong * 2)()
     check(lib().mhd_selfcheck_division(int(stride), m))
     return int(m[0]), int(m[1])
+
+
+def selfcheck_limiter(p, m):
+    """(out, out_nb): the FP32 van Leer limiter of the sweeps on the device
+    for the pairs (p[i], m[i]), in its branching form (tiles) and its
+    branch-free form (streamed tvdb)."""
+    p = np.ascontiguousarray(p, dtype=np.float32)
+    m = np.ascontiguousarray(m, dtype=np.float32)
+    if p.shape != m.shape or p.ndim != 1:
+        raise ValueError("selfcheck_limiter: two 1-D arrays of one length")
+    out, out_nb = np.empty_like(p), np.empty_like(p)
+    check(lib().mhd_selfcheck_limiter(p.ctypes.data, m.ctypes.data, out.ctypes.data, out_nb.ctypes.data, p.size))
+    return out, out_nb
 
 
 class Simulation:

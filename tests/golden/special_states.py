@@ -40,10 +40,20 @@ def special_state(name, nx, ny, nz, seed):
         U[cells[0], 1] = F32(2.0 ** 61)
         U[cells[0], 4] = F32(2.0 ** 63)
         U[cells[1], 2] = F32(-(2.0 ** 60))
+    elif name == "huge_field":         # face fields and momenta whose products, differences
+        B = b.reshape(-1, 3)           # and limiter sums overflow FP32 in the sweeps
+        B[cells[0], :] = F32(2.0 ** 64)
+        B[cells[1], 1] = F32(-(2.0 ** 64))
+        U[cells[0], 1:4] = F32(2.0 ** 63)
+        U[cells[1], 1:4] = F32(-(2.0 ** 63))
     else:
         raise KeyError(name)
     return u, b
 
+
+# states outside the committed golden vectors (pinned against the reference
+# build directly)
+EXTRA_CASES = ["huge_field"]
 
 CFL_CASES = ["neg_density", "neg_density_many", "zero_density", "nan_energy", "nan_last_cell", "inf_density",
              "tiny_density", "huge_momentum"]

@@ -58,6 +58,53 @@ def test_shared_divisor_quotient_is_ieee(M):
     assert bad_q == 0 and bad_rcp == 0
 
 
+def limiter_pairs(rng):
+    """FP32 operand pairs of the van Leer limiter: random bit patterns (every
+    exponent, signs, subnormals, infinities, NaN), sums that overflow FP32,
+    products on either side of the 2^-150 underflow tie, exact quotients and
+    near-equal operands"""
+    bits = rng.integers(0, 1 << 32, size=(200000, 2), dtype=np.uint64).astype(np.uint32)
+    p, m = list(bits[:, 0].view(np.float32)), list(bits[:, 1].view(np.float32))
+    big = rng.uniform(1.0, 2.0, size=(5000, 2)) * 2.0 ** 127
+    sgn = np.where(rng.random(5000) < 0.5, -1.0, 1.0)
+    p += list(np.float32(big[:, 0] * sgn))
+    m += list(np.float32(big[:, 1] * sgn))
+    for x in rng.random(5000):
+        a = np.float32(np.ldexp(1 + x, -75))
+        b = np.float32(2.0 ** -150 / float(a))
+        for d in (-1, 0, 1):
+            p.append(a)
+            m.append(np.float32(b * np.float32(1 + d * 2.0 ** -23)))
+    near = rng.uniform(0.5, 2.0, 5000).astype(np.float32)
+    p += list(near)
+    m += list(np.float32(near * np.float32(1 + 2.0 ** -22)))
+    special = [np.inf, -np.inf, np.nan, 0.0, -0.0, 2.0 ** -149, 3.4028235e38, 1.0, 3.0, 0.75]
+    for a in special:
+        for b in special:
+            p.append(np.float32(a))
+            m.append(np.float32(b))
+    return np.array(p, np.float32), np.array(m, np.float32)
+
+
+def test_limiter_forms_equal_the_reference_expression(M):
+    # tvd1.c:71-81, tvdb.c:72-76: d = 2.0*p*m/(p+m) in double with p+m summed
+    # in FP32, where the FP32 p*m > 0, else 0; both device forms (the test
+    # read off the exact FP64 numerator, the refined reciprocal, the IEEE
+    # division for an infinite sum, the clamp of the branch-free form) must
+    # give it bit for bit, NaN where it is NaN
+    p, m = limiter_pairs(np.random.default_rng(23))
+    with np.errstate(all="ignore"):
+        taken = (p * m) > np.float32(0)
+        q = (2.0 * p.astype(np.float64) * m.astype(np.float64) / (p + m).astype(np.float64)).astype(np.float32)
+    want = np.where(taken, q, np.float32(0))
+    out, out_nb = M.selfcheck_limiter(p, m)
+    print("limiter pairs", p.size, "taken", int(taken.sum()), flush=True)
+    for got in (out, out_nb):
+        nan = np.isnan(want)
+        assert np.array_equal(np.isnan(got), nan)
+        assert np.array_equal(got[~nan].view(np.uint32), want[~nan].view(np.uint32))
+
+
 SHAPES = [(16, 4, 3), (48, 5, 4), (64, 3, 3), (96, 3, 5), (128, 2, 3), (192, 2, 2), (256, 3, 2), (320, 1, 2),
           (384, 2, 1), (512, 1, 2), (24, 1, 1), (40, 7, 1)]
 

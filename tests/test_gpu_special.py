@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 from conftest import same
-from golden.special_states import CFL_CASES, special_state
+from golden.special_states import CFL_CASES, EXTRA_CASES, special_state
 
 pytestmark = pytest.mark.gpu
 F32 = np.float32
@@ -80,6 +80,44 @@ def test_fast_division_fallbacks(M, oracle, name, shape):
     # ... and whole steps of the device-resident path (the SoA sweeps in all
     # three frames), dt from the CFL of that state
     sim = M.Simulation(*shape)
+    sim.set_state(u0, b0)
+    sim.step(2)
+    ug, bg = sim.get_state()
+    dg = sim.dts()
+    sim.close()
+    uo, bo = u0.copy(), b0.copy()
+    do = oracle.run(uo, bo, *shape, 2)
+    assert same(dg, do) and same(ug, uo) and same(bg, bo)
+
+
+@pytest.mark.parametrize("name", EXTRA_CASES + ["inf_density", "huge_momentum"])
+@pytest.mark.parametrize("shape", [(64, 8, 6), (24, 20, 9)])
+def test_overflow_states_single_ops(M, oracle, name, shape):
+    # FP32 products, differences and limiter sums that overflow inside one
+    # fluidx or advectbyzx: a single op leaves most pencils finite, so every
+    # inf and NaN position is checked against the oracle
+    print("overflow ops", name, shape, flush=True)
+    u0, b0 = special_state(name, *shape, seed=7)
+    dt = F32(0.2)
+    for op in ("fluidx", "advectbyzx"):
+        du, db = M.DeviceArray.from_host(u0), M.DeviceArray.from_host(b0)
+        getattr(M, op)(du, db, *shape, M.DeviceArray.from_host(np.array([dt], F32)))
+        uo, bo = u0.copy(), b0.copy()
+        getattr(oracle, op)(uo, bo, *shape, dt)
+        assert same(du.to_host(), uo) and same(db.to_host(), bo), op
+
+
+@pytest.mark.parametrize("form", ["tiles", "two_pass", "fused"])
+@pytest.mark.parametrize("name", EXTRA_CASES + ["inf_density", "nan_energy", "huge_momentum", "tiny_density"])
+def test_special_states_every_advect_form(M, oracle, name, form):
+    # the y/z advectbyzx forms, forced on a small box (the automatic choice
+    # streams only large ones), on states with infinite, NaN, huge and tiny
+    # values: two whole steps against the oracle
+    shape = (64, 20, 9)
+    print("advect form", form, name, flush=True)
+    u0, b0 = special_state(name, *shape, seed=11)
+    sim = M.Simulation(*shape)
+    sim.set_advect_form(form)
     sim.set_state(u0, b0)
     sim.step(2)
     ug, bg = sim.get_state()

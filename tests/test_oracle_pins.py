@@ -18,7 +18,7 @@ import pytest
 
 import oracle_lib
 from conftest import same
-from golden.special_states import CFL_CASES, special_state
+from golden.special_states import CFL_CASES, EXTRA_CASES, special_state
 
 F32 = np.float32
 G = 5.0 / 3.0
@@ -59,6 +59,27 @@ def test_special_states_steps_equal_reference(oracle, name):
         da = oracle.run(ua, ba, n, n, n, 3)
         db = ref.run(ub, bb, n, n, n, 3)
         assert same(da, db) and same(ua, ub) and same(ba, bb)
+
+
+@needs_ref
+@pytest.mark.parametrize("name", EXTRA_CASES)
+def test_extra_states_equal_reference(oracle, name):
+    # states beyond the committed vectors (FP32 overflow inside the sweeps):
+    # the CFL, single ops on boxes of several shapes, and three whole steps
+    ref = oracle_lib.Reference()
+    for shape in ((8, 8, 8), (12, 6, 10), (20, 4, 6)):
+        u, b = special_state(name, *shape, seed=5)
+        assert f32_equal(oracle.calcfl(u, b, *shape), ref.calcfl(u, b, *shape))
+        for op in ("fluidx", "advectbyzx"):
+            ua, ba, ub, bb = u.copy(), b.copy(), u.copy(), b.copy()
+            getattr(oracle, op)(ua, ba, *shape, F32(0.2))
+            getattr(ref, op)(ub, bb, *shape, F32(0.2))
+            assert same(ua, ub) and same(ba, bb), (op, shape)
+    for n in (8, 10):
+        u, b = special_state(name, n, n, n, seed=5)
+        ua, ba, ub, bb = u.copy(), b.copy(), u.copy(), b.copy()
+        assert same(oracle.run(ua, ba, n, n, n, 3), ref.run(ub, bb, n, n, n, 3))
+        assert same(ua, ub) and same(ba, bb)
 
 
 def test_special_states_golden(oracle, golden):
