@@ -63,6 +63,12 @@ int mhd_selfcheck(double *rcp_f64_err);
  * mismatches[1] counts the floats of exponent [-40, 40] whose v_rcp_f32 is
  * not exactly the scaled, signed value at the mantissa.  Both must be 0. */
 int mhd_selfcheck_division(int stride, unsigned long long *mismatches);
+/* hardware property the freezing speed of the sweeps relies on (no
+ * counterpart in the reference): *mismatches counts the positive finite
+ * floats x whose shortened FP64 square root (one coupled Newton step and one
+ * residual correction from the reciprocal square root estimate) differs from
+ * the IEEE sqrt((double)x); exhaustive, a few milliseconds.  Must be 0. */
+int mhd_selfcheck_sqrt(unsigned long long *mismatches);
 /* the FP32 van Leer limiter as the sweeps evaluate it (no counterpart in the
  * reference; a check of the limiter shortcuts on the device): out[i] from
  * the branching form of the tiles, out_nb[i] from the branch-free form of the

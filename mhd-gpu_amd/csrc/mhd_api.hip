@@ -236,6 +236,22 @@ int mhd_selfcheck_division(int stride, unsigned long long *mismatches)
     return MHD_OK;
 }
 
+int mhd_selfcheck_sqrt(unsigned long long *mismatches)
+{
+    if (!mismatches) return fail(MHD_ERR_ARG, "mhd_selfcheck_sqrt: null argument");
+    unsigned long long *d = nullptr;
+    hipError_t e = hipMalloc(&d, sizeof *d);
+    if (e != hipSuccess) return hip_fail(e, "mhd_selfcheck_sqrt");
+    unsigned long long h = 0;
+    e = hipMemset(d, 0, sizeof *d);
+    if (e == hipSuccess) e = launch_sqrt_scan(d, 0);
+    if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof h, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "mhd_selfcheck_sqrt");
+    *mismatches = h;
+    return MHD_OK;
+}
+
 int mhd_selfcheck_limiter(const float *p, const float *m, float *out, float *out_nb, int n)
 {
     if (!p || !m || !out || !out_nb || n < 0) return fail(MHD_ERR_ARG, "mhd_selfcheck_limiter: bad argument");

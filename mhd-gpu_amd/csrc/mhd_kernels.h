@@ -53,6 +53,7 @@ hipError_t launch_copy_f32(float *dst, const float *src, long n, hipStream_t st)
 hipError_t launch_rcp_scan(unsigned long long *maxbits, hipStream_t st);
 hipError_t launch_div_scan(unsigned stride, unsigned long long *mismatches, hipStream_t st);
 hipError_t launch_limiter_eval(const float *p, const float *m, float *out, float *out_nb, int n, hipStream_t st);
+hipError_t launch_sqrt_scan(unsigned long long *mismatches, hipStream_t st);
 
 // which = 12 or 13: out-of-place transposes with the reference's component maps
 template <typename T>

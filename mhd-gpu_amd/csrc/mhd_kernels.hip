@@ -564,6 +564,26 @@ hipError_t launch_div_scan(unsigned stride, unsigned long long *mismatches, hipS
     return hipGetLastError();
 }
 
+// sqrt_float_valued against the IEEE double square root on every positive
+// finite float x (bit patterns 1 .. 0x7f7fffff), grid-stride
+__global__ void __launch_bounds__(256) sqrt_scan_kernel(unsigned long long *mismatches)
+{
+    const unsigned last = 0x7f7fffffu, step = gridDim.x * 256u;
+    unsigned long long bad = 0;
+    for (unsigned i = blockIdx.x * 256u + threadIdx.x + 1u; i <= last; i += step) {
+        const double x = (double)__uint_as_float(i);
+        bad += __double_as_longlong(sqrt_float_valued(x)) != __double_as_longlong(__builtin_sqrt(x));
+        if (i > last - step) break;
+    }
+    if (bad) atomicAdd(mismatches, bad);
+}
+
+hipError_t launch_sqrt_scan(unsigned long long *mismatches, hipStream_t st)
+{
+    hipLaunchKernelGGL(sqrt_scan_kernel, dim3(8192), dim3(256), 0, st, mismatches);
+    return hipGetLastError();
+}
+
 // Both FP32 forms of the van Leer limiter (tvd1.c:71-81, tvdb.c:72-76) on
 // given operand pairs: the branching one of the tiles and the branch-free
 // one of the streamed tvdb

@@ -46,6 +46,7 @@ SIGNATURES = {
     "mhd_selfcheck": (_I, [ctypes.POINTER(ctypes.c_double)]),
     "mhd_selfcheck_division": (_I, [ctypes.c_int, ctypes.POINTER(ctypes.c_ulonglong)]),
     "mhd_selfcheck_limiter": (_I, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
+    "mhd_selfcheck_sqrt": (_I, [ctypes.POINTER(ctypes.c_ulonglong)]),
     "mhd_fluidx": (_I, [_P, _P, _IP, _IP, _IP, _P]),
     "mhd_advectbyzx": (_I, [_P, _P, _IP, _IP, _IP, _P]),
     "mhd_advectbyzx_noalloc": (_I, [_P, _P, _IP, _IP, _IP, _P]),
@@ -265,6 +266,14 @@ def selfcheck_division(stride=1):
     m = (ctypes.c_ulonglong * 2)()
     check(lib().mhd_selfcheck_division(int(stride), m))
     return int(m[0]), int(m[1])
+
+
+def selfcheck_sqrt():
+    """mismatches of the shortened FP64 square root of the freezing speed
+    against the IEEE root over every positive finite float; must be 0."""
+    m = ctypes.c_ulonglong()
+    check(lib().mhd_selfcheck_sqrt(ctypes.byref(m)))
+    return int(m.value)
 
 
 def selfcheck_limiter(p, m):

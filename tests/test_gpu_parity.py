@@ -58,6 +58,15 @@ def test_shared_divisor_quotient_is_ieee(M):
     assert bad_q == 0 and bad_rcp == 0
 
 
+def test_freezing_speed_sqrt_is_ieee(M):
+    # the root in mhdflux's freezing speed (mhdflux.c:45) takes a float-valued
+    # argument, and its shortened sequence must give the IEEE double root on
+    # every positive finite float
+    bad = M.selfcheck_sqrt()
+    print("shortened sqrt mismatches over all positive floats: %d" % bad, flush=True)
+    assert bad == 0
+
+
 def limiter_pairs(rng):
     """FP32 operand pairs of the van Leer limiter: random bit patterns (every
     exponent, signs, subnormals, infinities, NaN), sums that overflow FP32,
