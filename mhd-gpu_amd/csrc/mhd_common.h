@@ -139,9 +139,9 @@ __device__ __forceinline__ float limiter_quotient_nb(float p, float m, double N)
 // IEEE FP64 square root for x = 0, +inf or x >= 2^-767: the compiler's own
 // sequence (reciprocal square root estimate, a coupled Newton step on the
 // root and the half reciprocal, two residual corrections; zero and +inf pass
-// through) without its rescaling of arguments below 2^-767.  The sweeps
-// take roots of float-valued doubles and of quotients of floats, never that
-// small.  Both sequences round correctly, so they agree on every argument.
+// through) without its rescaling of arguments below 2^-767.  The CFL speed
+// takes roots of FP64 quotients of floats, never that small.  Both
+// sequences round correctly, so they agree on every argument.
 __device__ __forceinline__ double sqrt_ranged(double x)
 {
     const double r = __builtin_amdgcn_rsq(x);
@@ -187,8 +187,9 @@ template <typename T> __device__ __forceinline__ double sqrt_of_fv(double x)
         return sqrt(x);
 }
 
-// van Leer harmonic mean: the test is a product in T, the value
-// 2*p*m/(p+m) in double with p+m summed in T (tvd1.c:71-81, tvdb.c:72-76).
+// van Leer harmonic mean: the test is the product p*m > 0 in T (for FP32
+// read off the exact FP64 numerator, limiter_taken), the value 2*p*m/(p+m)
+// in double with p+m summed in T (tvd1.c:71-81, tvdb.c:72-76).
 template <typename T> __device__ __forceinline__ T vanleer(T p, T m)
 {
     T d = (T)0;
