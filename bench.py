@@ -387,7 +387,8 @@ def main():
                          "precision: FP32 vs FP64 report on --ic at --n for --steps; "
                          "steplevel: the reference's loop through the step-level AoS entry points; "
                          "selfcheck: the exhaustive scans of the reciprocal instructions the exact "
-                         "FP32 division and limiter shortcuts rely on")
+                         "FP32 division and limiter shortcuts rely on, and of the shortened square "
+                         "root of float-valued arguments")
     ap.add_argument("--ic", choices=["random_phase", "turbulence"], default="random_phase")
     ap.add_argument("--seed", type=int, default=2)
     args = ap.parse_args()
@@ -419,9 +420,11 @@ def main():
         t0 = time.perf_counter()
         rcp64 = mhdgpu.selfcheck()
         bad_q, bad_rcp = mhdgpu.selfcheck_division(1)
+        bad_sqrt = mhdgpu.selfcheck_sqrt()
         print(json.dumps({"mode": "selfcheck", "rcp_f64_max_rel_err": rcp64,
                           "quotient_pairs": 1 << 46, "one_correction_quotient_mismatches": bad_q,
                           "rcp_f32_equivariance_mismatches": bad_rcp,
+                          "float_valued_sqrt_mismatches": bad_sqrt,
                           "wall_s": time.perf_counter() - t0}), flush=True)
         return
 
